@@ -1,0 +1,205 @@
+/*
+ * rbffd.h -- C ABI of librbffd, an MI355X (gfx950) engine for RBF-FD operator
+ * generation and application.  Drop-in for the operator path of
+ * RadialBasisFiniteDifferences.jl: the reference's Julia functions keep their
+ * signatures and ccall these entry points (citations are relative to the
+ * reference root).
+ *
+ * Conventions
+ *   - Points are interleaved Float64 pairs x0, y0, x1, y1, ... (the memory of a
+ *     Vector{SVector{2,Float64}}, test/poisson_test.jl:21-22).
+ *   - *_device entry points take device pointers and run on the context's
+ *     stream; they synchronise the stream before returning where they must read
+ *     a size or a status back (noted per function).  Host entry points take host
+ *     pointers and are fully synchronous.
+ *   - Every function returns RBFFD_OK (0) or an error code; rbffd_last_error()
+ *     gives the message of the calling thread's last failure.
+ *   - Indices inside the library are 0-based int32; the CSC outputs meant for
+ *     SparseMatrixCSC{Float64,Int64} are int64 with a caller-chosen base.
+ *   - One context per concurrent caller; a context is not shared between
+ *     threads without external locking.
+ */
+#ifndef RBFFD_H
+#define RBFFD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBFFD_VERSION 1
+
+/* status codes (the Julia shim maps them to the exceptions the reference throws) */
+#define RBFFD_OK 0
+#define RBFFD_ERR_ARG 1          /* invalid argument          -> ArgumentError           */
+#define RBFFD_ERR_UNSUPPORTED 2  /* no kernel for this shape  -> ArgumentError           */
+#define RBFFD_ERR_RUNTIME 3      /* HIP runtime failure       -> ErrorException          */
+#define RBFFD_ERR_SINGULAR 4     /* exactly singular stencil  -> LinearAlgebra.SingularException
+                                    (inv at src/interpolationmatrix.jl:8)               */
+#define RBFFD_ERR_NONFINITE 5    /* non-finite weights (NaN/Inf coordinates)            */
+#define RBFFD_ERR_UNDEFINED 6    /* node in no index set      -> UndefRefError
+                                    (idxs_x never filled, src/calculateneighbors.jl:8-41) */
+
+/* operator kinds: the right-hand-side columns of the reference's local solves */
+#define RBFFD_OP_E 0     /* evaluation E         src/generate_operator.jl:157,166 */
+#define RBFFD_OP_DX 1    /* d/dx                 :157,161 */
+#define RBFFD_OP_DY 2    /* d/dy                 :157,162 */
+#define RBFFD_OP_DXX 3   /* d2/dx2               :157,163 */
+#define RBFFD_OP_DYY 4   /* d2/dy2               :157,164 */
+#define RBFFD_OP_DXY 5   /* d2/dxdy              :157,165 */
+#define RBFFD_OP_XK 6    /* d^k/dx^k             src/hyperviscosity_operator.jl:155,159 */
+#define RBFFD_OP_YK 7    /* d^k/dy^k             src/hyperviscosity_operator.jl:155,160 */
+#define RBFFD_OP_LAP 8   /* Laplacian, one solve (equals Dxx + Dyy by linearity) */
+#define RBFFD_OP_LAPK 9  /* Laplacian^k with the per-axis stencil scaling (no reference counterpart) */
+#define RBFFD_MAX_OPS 10
+
+/* precision of the local solves */
+#define RBFFD_PREC_FP64 0   /* FP64 Gauss-Jordan with partial pivoting */
+#define RBFFD_PREC_MIXED 1  /* FP32 Gauss-Jordan inverse + FP64 refinement, FP64 re-solve where it
+                               does not converge */
+
+/* value types for rbffd_spmv_device */
+#define RBFFD_F64 0
+#define RBFFD_F32 1
+
+typedef struct rbffd_context rbffd_context;
+
+/* Boundary index sets of generate_operator method 2 (src/generate_operator.jl:192-193,
+ * src/calculateneighbors.jl:1-97).  Marker b owns bc_idx[bc_ptr[b] .. bc_ptr[b+1])
+ * and ghost_idx[ghost_ptr[b] .. ghost_ptr[b+1]). */
+typedef struct rbffd_boundary {
+    int32_t index_base;        /* 0 or 1 (Julia) */
+    int64_t n_in;
+    const int64_t *in_idx;     /* X_idx_in */
+    int32_t n_markers;
+    const int64_t *bc_ptr;     /* n_markers + 1 offsets into bc_idx (0-based offsets) */
+    const int64_t *bc_idx;     /* X_idx_bc[b] concatenated */
+    const int64_t *ghost_ptr;  /* n_markers + 1 offsets into ghost_idx */
+    const int64_t *ghost_idx;  /* X_idx_bc_g[b] concatenated */
+} rbffd_boundary;
+
+/* What one generation call did. */
+typedef struct rbffd_stats {
+    int64_t rows;          /* output rows (= number of Y nodes) */
+    int64_t singular;      /* rows whose system had an exactly zero pivot */
+    int64_t nonfinite;     /* rows with non-finite weights */
+    int64_t refined_fallback; /* mixed precision: rows re-solved in FP64 */
+    double ms_knn, ms_solve, ms_fallback;  /* device time per phase (HIP events) */
+} rbffd_stats;
+
+/* ---------------------------------------------------------------- context */
+int rbffd_version(void);
+const char *rbffd_last_error(void);
+int rbffd_create(int device, rbffd_context **ctx);
+int rbffd_destroy(rbffd_context *ctx);
+/* Run on this HIP stream from now on (NULL: the context's own stream). */
+int rbffd_set_stream(rbffd_context *ctx, void *hip_stream);
+int rbffd_synchronize(rbffd_context *ctx);
+
+/* ------------------------------------------------------ device plumbing
+ * No reference counterpart (the reference has no device).  For callers without
+ * their own GPU allocator (the Julia shim, the Python mirror): device memory,
+ * copies on the context stream (synchronous), and HIP-event timing of the
+ * context stream.  rbffd_timer_stop returns the milliseconds between the last
+ * rbffd_timer_start and now. */
+int rbffd_device_malloc(rbffd_context *ctx, int64_t bytes, void **ptr);
+int rbffd_device_free(rbffd_context *ctx, void *ptr);
+int rbffd_memcpy_h2d(rbffd_context *ctx, void *dst, const void *src, int64_t bytes);
+int rbffd_memcpy_d2h(rbffd_context *ctx, void *dst, const void *src, int64_t bytes);
+int rbffd_memcpy_d2d(rbffd_context *ctx, void *dst, const void *src, int64_t bytes);
+int rbffd_memset_device(rbffd_context *ctx, void *dst, int32_t value, int64_t bytes);
+int rbffd_timer_start(rbffd_context *ctx);
+int rbffd_timer_stop(rbffd_context *ctx, float *ms);
+/* Name of the device, compute units, and the gfx target (e.g. "gfx950"). */
+int rbffd_device_info(rbffd_context *ctx, char *name, int32_t name_len, int32_t *compute_units,
+                      char *arch, int32_t arch_len);
+
+/* ---------------------------------------------------------------- kNN (K1)
+ * Replaces KDTree(X) + knn(tree, Q, k, true) (src/generate_operator.jl:43-47,
+ * src/hyperviscosity_operator.jl:54-58) and the masked searches of
+ * calculateneighbors (src/calculateneighbors.jl:16-94).
+ * idx: M x k nearest X nodes of every query (0-based, nearest first); d2: the
+ * squared distances (may be NULL).  Order: ascending dx*dx + dy*dy, ties by the
+ * lower index.  xgroup/qgroup (both or neither, may be NULL): a query with
+ * qgroup < 0 sees all X nodes, one with qgroup = g >= 0 sees nodes with xgroup 0
+ * or g.  Synchronises the stream (grid size is read back).  RBFFD_ERR_ARG if a
+ * query sees fewer than k nodes. */
+int rbffd_knn_device(rbffd_context *ctx, const double *X, int64_t N, const double *Q, int64_t M,
+                     int32_t k, const int32_t *xgroup, const int32_t *qgroup, int32_t *idx, double *d2);
+
+/* ------------------------------------------------ local weights (K2)
+ * One output row per Y node k: the weights of the stencil of X node y2x[k]
+ * (y2x NULL: Y is X and y2x[k] = k).  Replaces hot loops 1 and 2 of
+ * generate_operator (src/generate_operator.jl:55-166, method 3 :388-467) and
+ * hyperviscosity_operator (src/hyperviscosity_operator.jl:66-161).
+ * method: 1 (also method 2, same numerics) or 3.  ops/opk: nops operator kinds
+ * and their derivative order k (used by XK, YK, LAPK).
+ * Outputs: CSR rows of length n with sorted columns: colidx (M x n, may be
+ * NULL), vals[o] (M x n, FP64) for each operator, status (M, may be NULL:
+ * 0 ok, 1 singular, 2 non-finite, 3 refinement did not converge).
+ * Asynchronous. */
+int rbffd_weights_device(rbffd_context *ctx, const double *X, int64_t N, const double *Y, int64_t M,
+                         const int32_t *stencils, int32_t n, const int32_t *y2x, int32_t p,
+                         int32_t polydeg, int32_t method, int32_t nops, const int32_t *ops,
+                         const int32_t *opk, int32_t precision, int32_t *colidx, double *const *vals,
+                         int32_t *status);
+
+/* --------------------------------------------- whole generation pipeline
+ * kNN + nearest-X of every Y + weights, i.e. generate_operator /
+ * hyperviscosity_operator up to (not including) the CSC assembly.
+ * Y NULL selects method 3 (generate_operator(X, p, n, polydeg),
+ * src/generate_operator.jl:354).  bnd non-NULL selects method 2.
+ * stencils_out (N x n) and y2x_out (M) may be NULL.  Mixed precision re-solves
+ * unconverged rows in FP64.  Synchronises; fills stats (may be NULL).
+ * Returns RBFFD_ERR_SINGULAR / RBFFD_ERR_NONFINITE when rows failed (outputs are
+ * still written). */
+int rbffd_generate_device(rbffd_context *ctx, const double *X, int64_t N, const double *Y, int64_t M,
+                          int32_t p, int32_t n, int32_t polydeg, const rbffd_boundary *bnd,
+                          int32_t nops, const int32_t *ops, const int32_t *opk, int32_t precision,
+                          int32_t *colidx, double *const *vals, int32_t *stencils_out,
+                          int32_t *y2x_out, rbffd_stats *stats);
+
+/* --------------------------------------------------------- CSR -> CSC (K3)
+ * The SparseMatrixCSC layout of sparse(rows, cols, vals) (src/generate_operator.jl:171-182):
+ * colptr (ncols + 1), rowval (M n, rows ascending inside a column), csc_vals[o]
+ * (M n) for each of nops value arrays sharing the pattern.  Explicit zeros are
+ * kept.  Indices offset by index_base (1 for Julia).  ncols_used (may be NULL)
+ * receives 1 + the largest column referenced, the column count sparse() infers.
+ * Synchronises. */
+int rbffd_csr_to_csc_device(rbffd_context *ctx, int64_t M, int32_t n, int64_t ncols,
+                            const int32_t *colidx, int32_t nops, const double *const *csr_vals,
+                            int64_t *colptr, int64_t *rowval, double *const *csc_vals,
+                            int32_t index_base, int64_t *ncols_used);
+
+/* ----------------------------------------------------------- SpMV/SpMM (K4)
+ * y = alpha A x + beta y for F fields (1, 2, 4, 8) stored node-major
+ * (x[col F + f]); A is CSR with int32 columns and FP64 or FP32 values
+ * (accumulated in FP64).  uniform_n > 0: every row has uniform_n entries and
+ * rowptr may be NULL.  Replaces SparseMatrixCSC * Vector in the reference's
+ * harnesses (test/poisson_test.jl:121, examples/adv_diff_test.jl:151-152).
+ * Asynchronous. */
+int rbffd_spmv_device(rbffd_context *ctx, int64_t M, const int64_t *rowptr, const int32_t *col,
+                      const void *vals, int32_t vals_type, int32_t F, const double *x, double *y,
+                      double alpha, double beta, int32_t uniform_n);
+
+/* ------------------------------------------------ host entry point (Julia)
+ * generate_operator / hyperviscosity_operator with host arrays, returning the
+ * SparseMatrixCSC{Float64,Int64} arrays of every requested operator
+ * (src/generate_operator.jl:29,192,354; src/hyperviscosity_operator.jl:26,177,314).
+ * Y NULL: method 3.  bnd non-NULL: method 2.  Outputs (caller-allocated, sizes
+ * known before the call): colptr (N + 1), rowval (M n), nzval[o] (M n); all
+ * operators share colptr/rowval.  index_base 1 gives Julia's 1-based arrays.
+ * *ncols receives the column count sparse() infers (max referenced column);
+ * the row count is M.  Synchronous. */
+int rbffd_generate_operator(rbffd_context *ctx, const double *X, int64_t N, const double *Y, int64_t M,
+                            int32_t p, int32_t n, int32_t polydeg, const rbffd_boundary *bnd,
+                            int32_t nops, const int32_t *ops, const int32_t *opk, int32_t precision,
+                            int32_t index_base, int64_t *colptr, int64_t *rowval, double *const *nzval,
+                            int64_t *ncols, rbffd_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBFFD_H */

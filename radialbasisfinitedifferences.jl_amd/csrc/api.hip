@@ -1,0 +1,652 @@
+// api.hip -- the C ABI of librbffd (declared in include/rbffd.h): context,
+// device workspace, the generation pipeline (kNN -> nearest X of every Y ->
+// local solves -> FP64 re-solve of unconverged rows), CSR -> CSC and SpMV.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "rbffd.h"
+
+using rbffd::Error;
+
+struct rbffd_context {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    struct Buf { void *p = nullptr; size_t bytes = 0; };
+    Buf bufs[32];
+    hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+enum Slot {
+    S_BBOX_PART, S_BBOX, S_COUNTS, S_START, S_CELL_OF, S_FILL, S_CELL_XY, S_CELL_IDX, S_CELL_GRP,
+    S_SCAN_TMP, S_FLAG, S_XGROUP, S_QGROUP, S_STENCILS, S_Y2X, S_STATUS, S_ROWS, S_CSC_COUNTS,
+    S_CSC_PTR, S_CSC_FILL, S_CSC_PERM, S_CSC_ROW, S_H_X, S_H_Y, S_H_COL, S_H_VALS, S_H_CVALS,
+    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2
+};
+
+template <class F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return RBFFD_OK;
+    } catch (const Error &e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return RBFFD_ERR_RUNTIME;
+    } catch (...) {
+        g_last_error = "unknown failure";
+        return RBFFD_ERR_RUNTIME;
+    }
+}
+
+void *ws(rbffd_context *c, int slot, size_t bytes)
+{
+    auto &b = c->bufs[slot];
+    if (bytes == 0) bytes = 16;
+    if (b.bytes < bytes) {
+        if (b.p) RBFFD_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+        size_t want = bytes + bytes / 8;
+        RBFFD_HIP(hipMalloc(&b.p, want));
+        b.bytes = want;
+    }
+    return b.p;
+}
+
+void check_ctx(rbffd_context *c)
+{
+    if (!c) throw Error{RBFFD_ERR_ARG, "null context"};
+    RBFFD_HIP(hipSetDevice(c->device));
+}
+
+// ------------------------------------------------------------ small kernels
+__global__ void count_negative_kernel(const int *v, long n, int *flag)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n && v[i] < 0) atomicAdd(flag, 1);
+}
+
+__global__ void max_kernel(const int *v, long n, int *out)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) atomicMax(out, v[i]);
+}
+
+__global__ void status_count_kernel(const int *status, long n, int *counts)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) {
+        int s = status[i];
+        if (s > 0 && s < 4) atomicAdd(&counts[s], 1);
+    }
+}
+
+__global__ void collect_kernel(const int *status, long n, int code, int *list, int *counter)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n && status[i] == code) list[atomicAdd(counter, 1)] = (int)i;
+}
+
+inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+int read_flag(rbffd_context *c, const int *dflag)
+{
+    int h = 0;
+    RBFFD_HIP(hipMemcpyAsync(&h, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    RBFFD_HIP(hipStreamSynchronize(c->stream));
+    return h;
+}
+
+// ------------------------------------------------------------ kNN
+void knn_impl(rbffd_context *c, const double *X, long N, const double *Q, long M, int k,
+              const int *xgroup, const int *qgroup, int *idx, double *d2)
+{
+    if (N < 1) throw Error{RBFFD_ERR_ARG, "knn: no points"};
+    if (k < 1 || k > N) throw Error{RBFFD_ERR_ARG, "knn: k must be in 1..number of points"};
+    if ((xgroup == nullptr) != (qgroup == nullptr))
+        throw Error{RBFFD_ERR_ARG, "knn: xgroup and qgroup go together"};
+    if (M <= 0) return;
+    hipStream_t s = c->stream;
+    const int nb = (int)std::min<long>(1024, (N + 255) / 256);
+    double *part = (double *)ws(c, S_BBOX_PART, sizeof(double) * 4 * nb);
+    double *box = (double *)ws(c, S_BBOX, sizeof(double) * 4);
+    rbffd::launch_bbox(X, N, box, part, nb, s);
+    double hb[4];
+    RBFFD_HIP(hipMemcpyAsync(hb, box, sizeof(hb), hipMemcpyDeviceToHost, s));
+    RBFFD_HIP(hipStreamSynchronize(s));
+    if (!(std::isfinite(hb[0]) && std::isfinite(hb[1]) && std::isfinite(hb[2]) && std::isfinite(hb[3])))
+        throw Error{RBFFD_ERR_NONFINITE, "knn: non-finite coordinates"};
+
+    const double w = hb[2] - hb[0], h = hb[3] - hb[1];
+    const double ppc = std::max(2.0, k / 6.0);  // points per cell
+    rbffd::GridParams g{};
+    g.x0 = hb[0];
+    g.y0 = hb[1];
+    double cs;
+    if (w > 0 && h > 0) cs = sqrt(w * h * ppc / (double)N);
+    else cs = std::max(w, h) * ppc / (double)N;
+    if (!(cs > 0)) cs = 1.0;
+    long gx, gy;
+    for (;;) {
+        gx = w > 0 ? std::min<long>((long)(w / cs) + 1, 1L << 16) : 1;
+        gy = h > 0 ? std::min<long>((long)(h / cs) + 1, 1L << 16) : 1;
+        if (gx * gy <= 4 * N + 64) break;
+        cs *= 1.25;
+    }
+    g.gx = (int)gx;
+    g.gy = (int)gy;
+    g.csx = w > 0 ? w / (double)gx * (1.0 + 1e-12) : 1.0;
+    g.csy = h > 0 ? h / (double)gy * (1.0 + 1e-12) : 1.0;
+    const long ncell = gx * gy;
+
+    int *counts = (int *)ws(c, S_COUNTS, sizeof(int) * (ncell + 1));
+    int *start = (int *)ws(c, S_START, sizeof(int) * (ncell + 1));
+    int *cell_of = (int *)ws(c, S_CELL_OF, sizeof(int) * N);
+    int *fill = (int *)ws(c, S_FILL, sizeof(int) * ncell);
+    double *cxy = (double *)ws(c, S_CELL_XY, sizeof(double) * 2 * N);
+    int *cidx = (int *)ws(c, S_CELL_IDX, sizeof(int) * N);
+    int *cgrp = xgroup ? (int *)ws(c, S_CELL_GRP, sizeof(int) * N) : nullptr;
+    RBFFD_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (ncell + 1), s));
+    RBFFD_HIP(hipMemsetAsync(fill, 0, sizeof(int) * ncell, s));
+    rbffd::launch_cell_count(X, N, g, cell_of, counts, s);
+    size_t tb = rbffd::scan_tmp_bytes(ncell + 1);
+    void *tmp = ws(c, S_SCAN_TMP, tb);
+    rbffd::launch_exclusive_scan(counts, start, ncell + 1, tmp, tb, s);
+    rbffd::launch_cell_scatter(X, xgroup, N, cell_of, start, fill, cxy, cidx, cgrp, s);
+    // X queries run in cell order so a wavefront walks neighbouring cells
+    const int *qorder = (Q == X && M == N) ? cidx : nullptr;
+    rbffd::launch_knn_query(g, start, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    RBFFD_HIP(hipGetLastError());
+
+    int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
+    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+    count_negative_kernel<<<nblk(M * (long)k), 256, 0, s>>>(idx, M * (long)k, flag);
+    if (read_flag(c, flag) != 0)
+        throw Error{RBFFD_ERR_ARG, "knn: a query sees fewer than k nodes"};
+}
+
+int n_monomials(int polydeg) { return (polydeg + 1) * (polydeg + 2) / 2; }
+
+void check_ops(int nops, const int *ops, const int *opk)
+{
+    if (nops < 1 || nops > RBFFD_MAX_OPS) throw Error{RBFFD_ERR_ARG, "nops must be in 1..10"};
+    for (int o = 0; o < nops; ++o) {
+        if (ops[o] < 0 || ops[o] > RBFFD_OP_LAPK) throw Error{RBFFD_ERR_ARG, "unknown operator kind"};
+        if ((ops[o] == RBFFD_OP_XK || ops[o] == RBFFD_OP_YK || ops[o] == RBFFD_OP_LAPK) &&
+            (!opk || opk[o] < 0 || opk[o] > 12))
+            throw Error{RBFFD_ERR_ARG, "derivative order k must be in 0..12"};
+    }
+}
+
+void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, long M, const int *stencils,
+                  int n, const int *y2x, const int *rows, long nrows, int p, int polydeg, int method,
+                  int nops, const int *ops, const int *opk, int precision, int *colidx,
+                  double *const *vals, int *status)
+{
+    (void)N;
+    if (method < 1 || method > 3) throw Error{RBFFD_ERR_ARG, "method must be 1, 2 or 3"};
+    if (p < 1) throw Error{RBFFD_ERR_ARG, "PHS power must be >= 1"};
+    if (polydeg < 0 || polydeg > 8) throw Error{RBFFD_ERR_ARG, "polynomial degree must be in 0..8"};
+    check_ops(nops, ops, opk);
+    if (precision != RBFFD_PREC_FP64 && precision != RBFFD_PREC_MIXED)
+        throw Error{RBFFD_ERR_ARG, "unknown precision"};
+    const int P = n_monomials(polydeg);
+    if (!rbffd::solve_supported(n, P, precision))
+        throw Error{RBFFD_ERR_UNSUPPORTED, "no compiled kernel for n=" + std::to_string(n) +
+                                               ", polydeg=" + std::to_string(polydeg) +
+                                               (precision ? " (mixed precision)" : "")};
+    // at most kMaxRhs right-hand sides per launch; more operators run in chunks
+    for (int o0 = 0; o0 < nops; o0 += rbffd::kMaxRhs) {
+        const int cnt = std::min(rbffd::kMaxRhs, nops - o0);
+        rbffd::SolveArgs a{};
+        a.X = X;
+        a.Y = Y;
+        a.stencils = stencils;
+        a.y2x = y2x;
+        a.rows = rows;
+        a.ntasks = rows ? nrows : M;
+        a.n = n;
+        a.P = P;
+        a.m = p;
+        a.method = method == 3 ? 3 : 1;
+        a.nops = cnt;
+        for (int o = 0; o < cnt; ++o) {
+            a.op_kind[o] = ops[o0 + o];
+            a.op_k[o] = opk ? opk[o0 + o] : 0;
+            a.vals[o] = vals[o0 + o];
+        }
+        a.colidx = o0 == 0 ? colidx : nullptr;
+        a.status = status;
+        a.status_accumulate = o0 > 0;
+        a.max_iter = 8;
+        a.tol = 1e-14;
+        rbffd::launch_solve(a, precision, c->stream);
+        RBFFD_HIP(hipGetLastError());
+    }
+}
+
+// group ids that restate calculateneighbors' masks (src/calculateneighbors.jl:16-42, 83-87)
+void boundary_groups(const rbffd_boundary *b, long N, std::vector<int> &xg, std::vector<int> &qg)
+{
+    xg.assign(N, -2);
+    qg.assign(N, -1);
+    std::vector<char> seen(N, 0);
+    const long base = b->index_base;
+    auto at = [&](int64_t v) -> long {
+        long i = (long)v - base;
+        if (i < 0 || i >= N) throw Error{RBFFD_ERR_ARG, "boundary index out of range"};
+        return i;
+    };
+    for (int64_t t = 0; t < b->n_in; ++t) { long i = at(b->in_idx[t]); xg[i] = 0; seen[i] = 1; }
+    for (int m = 0; m < b->n_markers; ++m)
+        for (int64_t t = b->bc_ptr[m]; t < b->bc_ptr[m + 1]; ++t) { long i = at(b->bc_idx[t]); xg[i] = 0; seen[i] = 1; }
+    for (int m = 0; m < b->n_markers; ++m) {
+        for (int64_t t = b->ghost_ptr[m]; t < b->ghost_ptr[m + 1]; ++t) {
+            long i = at(b->ghost_idx[t]);
+            xg[i] = m + 1;
+            qg[i] = m + 1;
+            seen[i] = 1;
+        }
+        for (int64_t t = b->bc_ptr[m]; t < b->bc_ptr[m + 1]; ++t) qg[at(b->bc_idx[t])] = m + 1;
+    }
+    // interior queries search the full tree (src/calculateneighbors.jl:83-87)
+    for (int64_t t = 0; t < b->n_in; ++t) qg[at(b->in_idx[t])] = -1;
+    for (long i = 0; i < N; ++i)
+        if (!seen[i])
+            throw Error{RBFFD_ERR_UNDEFINED, "node " + std::to_string(i + base) +
+                                                 " is in no index set, so it has no stencil"};
+}
+
+void generate_impl(rbffd_context *c, const double *X, long N, const double *Y, long M, int p, int n,
+                   int polydeg, const rbffd_boundary *bnd, int nops, const int *ops, const int *opk,
+                   int precision, int *colidx, double *const *vals, int *stencils_out, int *y2x_out,
+                   rbffd_stats *stats, int *status_codes)
+{
+    hipStream_t s = c->stream;
+    const bool m3 = (Y == nullptr);
+    if (m3) { Y = X; M = N; }
+    if (n < 1 || n > N) throw Error{RBFFD_ERR_ARG, "stencil size must be in 1..number of X nodes"};
+    if (m3 && bnd) throw Error{RBFFD_ERR_ARG, "boundary sets need Y"};
+    for (int e = 0; e < 8; ++e)
+        if (!c->ev[e]) RBFFD_HIP(hipEventCreate(&c->ev[e]));
+    RBFFD_HIP(hipEventRecord(c->ev[0], s));
+
+    int *st = stencils_out ? stencils_out : (int *)ws(c, S_STENCILS, sizeof(int) * N * (size_t)n);
+    if (bnd) {
+        std::vector<int> xg, qg;
+        boundary_groups(bnd, N, xg, qg);
+        int *dxg = (int *)ws(c, S_XGROUP, sizeof(int) * N);
+        int *dqg = (int *)ws(c, S_QGROUP, sizeof(int) * N);
+        RBFFD_HIP(hipMemcpyAsync(dxg, xg.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
+        RBFFD_HIP(hipMemcpyAsync(dqg, qg.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
+        knn_impl(c, X, N, X, N, n, dxg, dqg, st, nullptr);
+    } else {
+        knn_impl(c, X, N, X, N, n, nullptr, nullptr, st, nullptr);
+    }
+    const int *y2x = nullptr;
+    if (!m3 && Y != X) {
+        int *d = y2x_out ? y2x_out : (int *)ws(c, S_Y2X, sizeof(int) * M);
+        // nearest X node of every Y node over all of X (src/generate_operator.jl:47,
+        // src/calculateneighbors.jl:90-94)
+        knn_impl(c, X, N, Y, M, 1, nullptr, nullptr, d, nullptr);
+        y2x = d;
+    } else if (y2x_out) {
+        throw Error{RBFFD_ERR_ARG, "y2x_out needs Y distinct from X"};
+    }
+    RBFFD_HIP(hipEventRecord(c->ev[1], s));
+
+    int *status = (int *)ws(c, S_STATUS, sizeof(int) * M);
+    weights_impl(c, X, N, Y, M, st, n, y2x, nullptr, 0, p, polydeg, m3 ? 3 : 1, nops, ops, opk, precision,
+                 colidx, vals, status);
+    RBFFD_HIP(hipEventRecord(c->ev[2], s));
+
+    int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
+    long nfallback = 0;
+    if (precision == RBFFD_PREC_MIXED) {
+        int *list = (int *)ws(c, S_ROWS, sizeof(int) * M);
+        RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+        collect_kernel<<<nblk(M), 256, 0, s>>>(status, M, 3, list, flag);
+        nfallback = read_flag(c, flag);
+        if (nfallback > 0)
+            weights_impl(c, X, N, Y, M, st, n, y2x, list, nfallback, p, polydeg, m3 ? 3 : 1, nops, ops, opk,
+                         RBFFD_PREC_FP64, colidx, vals, status);
+    }
+    RBFFD_HIP(hipEventRecord(c->ev[3], s));
+    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 4, s));
+    status_count_kernel<<<nblk(M), 256, 0, s>>>(status, M, flag);
+    int hc[4];
+    RBFFD_HIP(hipMemcpyAsync(hc, flag, sizeof(hc), hipMemcpyDeviceToHost, s));
+    RBFFD_HIP(hipStreamSynchronize(s));
+    if (status_codes) {
+        RBFFD_HIP(hipMemcpy(status_codes, status, sizeof(int) * M, hipMemcpyDeviceToHost));
+    }
+    if (stats) {
+        stats->rows = M;
+        stats->singular = hc[1];
+        stats->nonfinite = hc[2];
+        stats->refined_fallback = nfallback;
+        float t01 = 0, t12 = 0, t23 = 0;
+        RBFFD_HIP(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+        RBFFD_HIP(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+        RBFFD_HIP(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
+        stats->ms_knn = t01;
+        stats->ms_solve = t12;
+        stats->ms_fallback = t23;
+    }
+    if (hc[1] > 0)
+        throw Error{RBFFD_ERR_SINGULAR, std::to_string(hc[1]) + " stencil systems are exactly singular"};
+    if (hc[2] > 0)
+        throw Error{RBFFD_ERR_NONFINITE, std::to_string(hc[2]) + " rows have non-finite weights"};
+}
+
+void csc_impl(rbffd_context *c, long M, int n, long ncols, const int *colidx, int nops,
+              const double *const *csr_vals, int64_t *colptr, int64_t *rowval, double *const *csc_vals,
+              int base, int64_t *ncols_used)
+{
+    hipStream_t s = c->stream;
+    const long nnz = M * (long)n;
+    if (ncols < 1) throw Error{RBFFD_ERR_ARG, "csc: ncols must be >= 1"};
+    int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
+    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 2, s));
+    max_kernel<<<nblk(nnz), 256, 0, s>>>(colidx, nnz, flag);
+    count_negative_kernel<<<nblk(nnz), 256, 0, s>>>(colidx, nnz, flag + 1);
+    int hm[2] = {0, 0};
+    RBFFD_HIP(hipMemcpyAsync(hm, flag, sizeof(hm), hipMemcpyDeviceToHost, s));
+    RBFFD_HIP(hipStreamSynchronize(s));
+    if (hm[1] > 0 || hm[0] >= ncols) throw Error{RBFFD_ERR_ARG, "csc: column index out of range"};
+    if (ncols_used) *ncols_used = (int64_t)hm[0] + 1;
+
+    int *counts = (int *)ws(c, S_CSC_COUNTS, sizeof(int) * (ncols + 1));
+    int *ptr = (int *)ws(c, S_CSC_PTR, sizeof(int) * (ncols + 1));
+    int *fill = (int *)ws(c, S_CSC_FILL, sizeof(int) * ncols);
+    int *perm = (int *)ws(c, S_CSC_PERM, sizeof(int) * nnz);
+    int *row = (int *)ws(c, S_CSC_ROW, sizeof(int) * nnz);
+    RBFFD_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (ncols + 1), s));
+    RBFFD_HIP(hipMemsetAsync(fill, 0, sizeof(int) * ncols, s));
+    rbffd::launch_csc_count(colidx, nnz, counts, s);
+    size_t tb = rbffd::scan_tmp_bytes(ncols + 1);
+    void *tmp = ws(c, S_SCAN_TMP2, tb);
+    rbffd::launch_exclusive_scan(counts, ptr, ncols + 1, tmp, tb, s);
+    rbffd::launch_csc_scatter(colidx, M, n, ptr, fill, perm, row, s);
+    rbffd::launch_csc_sort_columns(ptr, ncols, row, perm, s);
+    for (int o = 0; o < nops; ++o) rbffd::launch_gather_f64(csr_vals[o], perm, nnz, csc_vals[o], s);
+    rbffd::launch_i32_to_i64(ptr, ncols + 1, colptr, base, s);
+    rbffd::launch_i32_to_i64(row, nnz, rowval, base, s);
+    RBFFD_HIP(hipGetLastError());
+    RBFFD_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int rbffd_version(void) { return RBFFD_VERSION; }
+
+const char *rbffd_last_error(void) { return g_last_error.c_str(); }
+
+int rbffd_create(int device, rbffd_context **out)
+{
+    return guarded([&] {
+        if (!out) throw Error{RBFFD_ERR_ARG, "null output"};
+        int count = 0;
+        RBFFD_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw Error{RBFFD_ERR_ARG, "no such device"};
+        auto *c = new rbffd_context();
+        c->device = device;
+        RBFFD_HIP(hipSetDevice(device));
+        RBFFD_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+        c->stream = c->own_stream;
+        *out = c;
+    });
+}
+
+int rbffd_destroy(rbffd_context *c)
+{
+    return guarded([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (auto &b : c->bufs)
+            if (b.p) (void)hipFree(b.p);
+        for (auto &e : c->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+        delete c;
+    });
+}
+
+int rbffd_set_stream(rbffd_context *c, void *stream)
+{
+    return guarded([&] {
+        check_ctx(c);
+        c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    });
+}
+
+int rbffd_synchronize(rbffd_context *c)
+{
+    return guarded([&] {
+        check_ctx(c);
+        RBFFD_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int rbffd_device_malloc(rbffd_context *c, int64_t bytes, void **ptr)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!ptr || bytes < 0) throw Error{RBFFD_ERR_ARG, "bad allocation request"};
+        *ptr = nullptr;
+        RBFFD_HIP(hipMalloc(ptr, (size_t)(bytes > 0 ? bytes : 16)));
+    });
+}
+
+int rbffd_device_free(rbffd_context *c, void *ptr)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (ptr) {
+            RBFFD_HIP(hipStreamSynchronize(c->stream));
+            RBFFD_HIP(hipFree(ptr));
+        }
+    });
+}
+
+static void copy_sync(rbffd_context *c, void *dst, const void *src, int64_t bytes, hipMemcpyKind kind)
+{
+    check_ctx(c);
+    if (bytes < 0 || ((!dst || !src) && bytes > 0)) throw Error{RBFFD_ERR_ARG, "bad copy"};
+    if (bytes == 0) return;
+    RBFFD_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, kind, c->stream));
+    RBFFD_HIP(hipStreamSynchronize(c->stream));
+}
+
+int rbffd_memcpy_h2d(rbffd_context *c, void *dst, const void *src, int64_t bytes)
+{
+    return guarded([&] { copy_sync(c, dst, src, bytes, hipMemcpyHostToDevice); });
+}
+
+int rbffd_memcpy_d2h(rbffd_context *c, void *dst, const void *src, int64_t bytes)
+{
+    return guarded([&] { copy_sync(c, dst, src, bytes, hipMemcpyDeviceToHost); });
+}
+
+int rbffd_memcpy_d2d(rbffd_context *c, void *dst, const void *src, int64_t bytes)
+{
+    return guarded([&] { copy_sync(c, dst, src, bytes, hipMemcpyDeviceToDevice); });
+}
+
+int rbffd_memset_device(rbffd_context *c, void *dst, int32_t value, int64_t bytes)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!dst || bytes < 0) throw Error{RBFFD_ERR_ARG, "bad memset"};
+        RBFFD_HIP(hipMemsetAsync(dst, value, (size_t)bytes, c->stream));
+        RBFFD_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int rbffd_timer_start(rbffd_context *c)
+{
+    return guarded([&] {
+        check_ctx(c);
+        for (int e = 6; e < 8; ++e)
+            if (!c->ev[e]) RBFFD_HIP(hipEventCreate(&c->ev[e]));
+        RBFFD_HIP(hipEventRecord(c->ev[6], c->stream));
+    });
+}
+
+int rbffd_timer_stop(rbffd_context *c, float *ms)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!ms || !c->ev[6]) throw Error{RBFFD_ERR_ARG, "timer not started"};
+        RBFFD_HIP(hipEventRecord(c->ev[7], c->stream));
+        RBFFD_HIP(hipEventSynchronize(c->ev[7]));
+        RBFFD_HIP(hipEventElapsedTime(ms, c->ev[6], c->ev[7]));
+    });
+}
+
+int rbffd_device_info(rbffd_context *c, char *name, int32_t name_len, int32_t *cus, char *arch, int32_t arch_len)
+{
+    return guarded([&] {
+        check_ctx(c);
+        hipDeviceProp_t p;
+        RBFFD_HIP(hipGetDeviceProperties(&p, c->device));
+        if (name && name_len > 0) { strncpy(name, p.name, name_len - 1); name[name_len - 1] = 0; }
+        if (cus) *cus = p.multiProcessorCount;
+        if (arch && arch_len > 0) { strncpy(arch, p.gcnArchName, arch_len - 1); arch[arch_len - 1] = 0; }
+    });
+}
+
+int rbffd_knn_device(rbffd_context *c, const double *X, int64_t N, const double *Q, int64_t M, int32_t k,
+                     const int32_t *xgroup, const int32_t *qgroup, int32_t *idx, double *d2)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!X || !Q || !idx) throw Error{RBFFD_ERR_ARG, "null pointer"};
+        knn_impl(c, X, N, Q, M, k, xgroup, qgroup, idx, d2);
+    });
+}
+
+int rbffd_weights_device(rbffd_context *c, const double *X, int64_t N, const double *Y, int64_t M,
+                         const int32_t *stencils, int32_t n, const int32_t *y2x, int32_t p, int32_t polydeg,
+                         int32_t method, int32_t nops, const int32_t *ops, const int32_t *opk,
+                         int32_t precision, int32_t *colidx, double *const *vals, int32_t *status)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!X || !Y || !stencils || !ops || !vals) throw Error{RBFFD_ERR_ARG, "null pointer"};
+        for (int o = 0; o < nops && o < RBFFD_MAX_OPS; ++o)
+            if (!vals[o]) throw Error{RBFFD_ERR_ARG, "null value array"};
+        weights_impl(c, X, N, Y, M, stencils, n, y2x, nullptr, 0, p, polydeg, method, nops, ops, opk, precision,
+                     colidx, vals, status);
+    });
+}
+
+int rbffd_generate_device(rbffd_context *c, const double *X, int64_t N, const double *Y, int64_t M, int32_t p,
+                          int32_t n, int32_t polydeg, const rbffd_boundary *bnd, int32_t nops,
+                          const int32_t *ops, const int32_t *opk, int32_t precision, int32_t *colidx,
+                          double *const *vals, int32_t *stencils_out, int32_t *y2x_out, rbffd_stats *stats)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!X || !ops || !vals) throw Error{RBFFD_ERR_ARG, "null pointer"};
+        check_ops(nops, ops, opk);
+        for (int o = 0; o < nops; ++o)
+            if (!vals[o]) throw Error{RBFFD_ERR_ARG, "null value array"};
+        generate_impl(c, X, N, Y, M, p, n, polydeg, bnd, nops, ops, opk, precision, colidx, vals, stencils_out,
+                      y2x_out, stats, nullptr);
+    });
+}
+
+int rbffd_csr_to_csc_device(rbffd_context *c, int64_t M, int32_t n, int64_t ncols, const int32_t *colidx,
+                            int32_t nops, const double *const *csr_vals, int64_t *colptr, int64_t *rowval,
+                            double *const *csc_vals, int32_t index_base, int64_t *ncols_used)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!colidx || !colptr || !rowval || (nops > 0 && (!csr_vals || !csc_vals)))
+            throw Error{RBFFD_ERR_ARG, "null pointer"};
+        if (M < 1 || n < 1) throw Error{RBFFD_ERR_ARG, "empty matrix"};
+        csc_impl(c, M, n, ncols, colidx, nops, csr_vals, colptr, rowval, csc_vals, index_base, ncols_used);
+    });
+}
+
+int rbffd_spmv_device(rbffd_context *c, int64_t M, const int64_t *rowptr, const int32_t *col, const void *vals,
+                      int32_t vals_type, int32_t F, const double *x, double *y, double alpha, double beta,
+                      int32_t uniform_n)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!col || !vals || !x || !y) throw Error{RBFFD_ERR_ARG, "null pointer"};
+        if (uniform_n <= 0 && !rowptr) throw Error{RBFFD_ERR_ARG, "rowptr needed for non-uniform rows"};
+        if (vals_type != RBFFD_F64 && vals_type != RBFFD_F32) throw Error{RBFFD_ERR_ARG, "unknown value type"};
+        rbffd::launch_spmv(M, rowptr, col, vals, vals_type == RBFFD_F32, F, x, y, alpha, beta, uniform_n,
+                           c->stream);
+        RBFFD_HIP(hipGetLastError());
+    });
+}
+
+int rbffd_generate_operator(rbffd_context *c, const double *X, int64_t N, const double *Y, int64_t M, int32_t p,
+                            int32_t n, int32_t polydeg, const rbffd_boundary *bnd, int32_t nops,
+                            const int32_t *ops, const int32_t *opk, int32_t precision, int32_t index_base,
+                            int64_t *colptr, int64_t *rowval, double *const *nzval, int64_t *ncols,
+                            rbffd_stats *stats)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!X || !ops || !colptr || !rowval || !nzval) throw Error{RBFFD_ERR_ARG, "null pointer"};
+        check_ops(nops, ops, opk);
+        if (N < 1) throw Error{RBFFD_ERR_ARG, "no X nodes"};
+        const bool m3 = (Y == nullptr);
+        const long Mr = m3 ? N : M;
+        const long nnz = Mr * (long)n;
+        hipStream_t s = c->stream;
+        double *dX = (double *)ws(c, S_H_X, sizeof(double) * 2 * N);
+        RBFFD_HIP(hipMemcpyAsync(dX, X, sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
+        double *dY = nullptr;
+        if (!m3) {
+            if (Y == X) {
+                dY = dX;
+            } else {
+                dY = (double *)ws(c, S_H_Y, sizeof(double) * 2 * Mr);
+                RBFFD_HIP(hipMemcpyAsync(dY, Y, sizeof(double) * 2 * Mr, hipMemcpyHostToDevice, s));
+            }
+        }
+        int *dcol = (int *)ws(c, S_H_COL, sizeof(int) * nnz);
+        double *dv = (double *)ws(c, S_H_VALS, sizeof(double) * nnz * nops);
+        double *dcv = (double *)ws(c, S_H_CVALS, sizeof(double) * nnz * nops);
+        int64_t *dptr = (int64_t *)ws(c, S_H_PTR64, sizeof(int64_t) * (N + 1));
+        int64_t *drow = (int64_t *)ws(c, S_H_ROW64, sizeof(int64_t) * nnz);
+        double *vp[RBFFD_MAX_OPS], *cvp[RBFFD_MAX_OPS];
+        for (int o = 0; o < nops; ++o) { vp[o] = dv + (size_t)o * nnz; cvp[o] = dcv + (size_t)o * nnz; }
+        generate_impl(c, dX, N, dY, Mr, p, n, polydeg, bnd, nops, ops, opk, precision, dcol, vp, nullptr,
+                      nullptr, stats, nullptr);
+        int64_t used = 0;
+        csc_impl(c, Mr, n, N, dcol, nops, vp, dptr, drow, cvp, index_base, &used);
+        if (ncols) *ncols = used;
+        RBFFD_HIP(hipMemcpyAsync(colptr, dptr, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost, s));
+        RBFFD_HIP(hipMemcpyAsync(rowval, drow, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost, s));
+        for (int o = 0; o < nops; ++o)
+            RBFFD_HIP(hipMemcpyAsync(nzval[o], cvp[o], sizeof(double) * nnz, hipMemcpyDeviceToHost, s));
+        RBFFD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"

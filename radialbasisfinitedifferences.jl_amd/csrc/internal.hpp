@@ -1,0 +1,98 @@
+// internal.hpp -- host-side declarations shared by the translation units of
+// librbffd (kernel launchers, argument blocks, error plumbing).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+namespace rbffd {
+
+constexpr int kMaxOps = 10;
+constexpr int kMaxRhs = 6;   // right-hand sides per solve launch
+
+// Failures inside the library are thrown as this and turned into a status code
+// plus a thread-local message at the C-ABI boundary.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define RBFFD_HIP(call)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::rbffd::Error{3, std::string(#call) + ": " + hipGetErrorString(e_)};   \
+    } while (0)
+
+// ---------------------------------------------------------------- kNN (K1)
+struct GridParams {
+    double x0, y0;    // lower-left corner
+    double csx, csy;  // cell sizes
+    int gx, gy;       // cells per axis
+};
+
+// Bounding box of X (device), result in box[4] = (xmin, ymin, xmax, ymax).
+void launch_bbox(const double *X, long N, double *box, double *partials, int nblocks, hipStream_t s);
+// Cell index of every point and the per-cell counts (counts must be zeroed).
+void launch_cell_count(const double *X, long N, GridParams g, int *cell_of, int *counts, hipStream_t s);
+// Exclusive scan of counts (ncell) into start (ncell + 1); tmp as returned by scan_tmp_bytes.
+size_t scan_tmp_bytes(long n);
+void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tmp_bytes, hipStream_t s);
+// Scatter points into cell order (fill must be zeroed).
+void launch_cell_scatter(const double *X, const int *xgroup, long N, const int *cell_of, const int *start,
+                         int *fill, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s);
+// Exact kNN of every query, ascending (d^2, index).
+void launch_knn_query(GridParams g, const int *cell_start, const double *cell_xy, const int *cell_idx,
+                      const int *cell_grp, const double *Q, const int *qgroup, const int *qorder, long M,
+                      int k, int *idx_out, double *d2_out, hipStream_t s);
+
+// ------------------------------------------------------- local solves (K2)
+struct SolveArgs {
+    const double *X;        // N x 2 interleaved
+    const double *Y;        // M x 2 interleaved
+    const int *stencils;    // N x n, 0-based, first = centre
+    const int *y2x;         // M: stencil of each Y row (NULL: collocated)
+    const int *rows;        // task -> Y row (NULL: identity)
+    long ntasks;
+    int n, P, m, method;    // stencil size, monomials, PHS power, 1 or 3
+    int nops;
+    int op_kind[kMaxOps];
+    int op_k[kMaxOps];
+    int *colidx;            // M x n (NULL: skip)
+    double *vals[kMaxOps];  // M x n each
+    int *status;            // M (NULL: skip)
+    int status_accumulate;  // keep the larger of the stored and the new status
+    // mixed precision refinement
+    int max_iter;
+    double tol;
+};
+
+// Which (n, P) shapes have compiled kernels.
+bool solve_supported(int n, int P, int precision);
+// precision: 0 = FP64 Gauss-Jordan, 1 = FP32 Gauss-Jordan inverse + FP64 refinement.
+// a.nops <= kMaxRhs.
+void launch_solve(const SolveArgs &a, int precision, hipStream_t s);
+
+// Compiled stencil shapes (n, P): the reference's tests (20,10) and (42,21),
+// and the synthetic node-set shapes (13,6), (21,10), (31,15), (43,21), (57,28).
+#define RBFFD_SHAPES(X) X(13, 6) X(20, 10) X(21, 10) X(31, 15) X(42, 21) X(43, 21) X(57, 28)
+#define RBFFD_DECLARE_SHAPE(nn, pp) void launch_solve_##nn##_##pp(const SolveArgs &a, int precision, hipStream_t s);
+RBFFD_SHAPES(RBFFD_DECLARE_SHAPE)
+#undef RBFFD_DECLARE_SHAPE
+
+// --------------------------------------------------- sparse (K3, K4)
+void launch_fill_rowptr(int64_t *rowptr, long M, int n, hipStream_t s);
+void launch_csc_count(const int *colidx, long nnz, int *counts, hipStream_t s);
+void launch_csc_scatter(const int *colidx, long M, int n, const int *colptr, int *fill, int *perm,
+                        int *rowval, hipStream_t s);
+void launch_csc_sort_columns(const int *colptr, long ncols, int *rowval, int *perm, hipStream_t s);
+void launch_gather_f64(const double *src, const int *perm, long nnz, double *dst, hipStream_t s);
+void launch_i32_to_i64(const int *src, long n, int64_t *dst, int64_t add, hipStream_t s);
+
+// y[M x F] = A x[N x F], CSR with int32 columns.  vals_f32 selects FP32 values
+// (accumulated in FP64).
+void launch_spmv(long M, const int64_t *rowptr, const int *col, const void *vals, int vals_f32, int F,
+                 const double *x, double *y, double alpha, double beta, int uniform_n, hipStream_t s);
+void launch_f64_to_f32(const double *src, long n, float *dst, hipStream_t s);
+
+}  // namespace rbffd

@@ -1,0 +1,251 @@
+// knn.hip -- K1: exact k-nearest-neighbour stencil selection on gfx950.
+//
+// Replaces the KDTree/knn calls of the reference (src/generate_operator.jl:43-47,
+// src/hyperviscosity_operator.jl:54-58, src/calculateneighbors.jl:16-94).
+// Points are bucketed into a uniform grid (counting sort by cell, cell-ordered
+// copies of the coordinates so a cell scan is one contiguous read); one thread
+// per query walks rings of cells outwards, skips cells farther than its current
+// k-th distance and stops once the unvisited region is provably farther.
+// The k best are kept sorted in registers.  Order is ascending squared
+// distance dx*dx + dy*dy (never fused), ties by lower index, so the result
+// does not depend on the order in which cells or points are visited.
+#include <hipcub/hipcub.hpp>
+#include <limits.h>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace rbffd {
+
+// ---------------------------------------------------------------- grid build
+__global__ void bbox_partial_kernel(const double *X, long N, double *partials)
+{
+    __shared__ double sm[4][256];
+    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        double x = X[2 * i], y = X[2 * i + 1];
+        a = fmin(a, x); b = fmin(b, y); c = fmax(c, x); d = fmax(d, y);
+    }
+    sm[0][threadIdx.x] = a; sm[1][threadIdx.x] = b; sm[2][threadIdx.x] = c; sm[3][threadIdx.x] = d;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sm[0][threadIdx.x] = fmin(sm[0][threadIdx.x], sm[0][threadIdx.x + o]);
+            sm[1][threadIdx.x] = fmin(sm[1][threadIdx.x], sm[1][threadIdx.x + o]);
+            sm[2][threadIdx.x] = fmax(sm[2][threadIdx.x], sm[2][threadIdx.x + o]);
+            sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partials[blockIdx.x * 4 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ void bbox_final_kernel(const double *partials, int nblocks, double *box)
+{
+    if (threadIdx.x != 0) return;
+    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY;
+    for (int i = 0; i < nblocks; ++i) {
+        a = fmin(a, partials[4 * i]); b = fmin(b, partials[4 * i + 1]);
+        c = fmax(c, partials[4 * i + 2]); d = fmax(d, partials[4 * i + 3]);
+    }
+    box[0] = a; box[1] = b; box[2] = c; box[3] = d;
+}
+
+void launch_bbox(const double *X, long N, double *box, double *partials, int nblocks, hipStream_t s)
+{
+    bbox_partial_kernel<<<nblocks, 256, 0, s>>>(X, N, partials);
+    bbox_final_kernel<<<1, 64, 0, s>>>(partials, nblocks, box);
+}
+
+__device__ __forceinline__ int cell_coord(double v, double v0, double cs, int g)
+{
+    int c = (int)((v - v0) / cs);
+    c = c < 0 ? 0 : c;
+    return c >= g ? g - 1 : c;
+}
+
+__global__ void cell_count_kernel(const double *X, long N, GridParams g, int *cell_of, int *counts)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int cx = cell_coord(X[2 * i], g.x0, g.csx, g.gx);
+    int cy = cell_coord(X[2 * i + 1], g.y0, g.csy, g.gy);
+    int c = cy * g.gx + cx;
+    cell_of[i] = c;
+    atomicAdd(&counts[c], 1);
+}
+
+void launch_cell_count(const double *X, long N, GridParams g, int *cell_of, int *counts, hipStream_t s)
+{
+    if (N <= 0) return;
+    cell_count_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, N, g, cell_of, counts);
+}
+
+size_t scan_tmp_bytes(long n)
+{
+    size_t bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int *)nullptr, (int *)nullptr, (int)n);
+    return bytes;
+}
+
+void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tmp_bytes, hipStream_t s)
+{
+    RBFFD_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, s));
+}
+
+__global__ void cell_scatter_kernel(const double *X, const int *xgroup, long N, const int *cell_of,
+                                    const int *start, int *fill, double *cell_xy, int *cell_idx,
+                                    int *cell_grp)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int c = cell_of[i];
+    int pos = start[c] + atomicAdd(&fill[c], 1);
+    cell_xy[2 * (long)pos] = X[2 * i];
+    cell_xy[2 * (long)pos + 1] = X[2 * i + 1];
+    cell_idx[pos] = (int)i;
+    if (cell_grp) cell_grp[pos] = xgroup[i];
+}
+
+void launch_cell_scatter(const double *X, const int *xgroup, long N, const int *cell_of, const int *start,
+                         int *fill, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s)
+{
+    if (N <= 0) return;
+    cell_scatter_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, xgroup, N, cell_of, start, fill,
+                                                                     cell_xy, cell_idx, cell_grp);
+}
+
+// ---------------------------------------------------------------- query
+__device__ __forceinline__ bool key_less(double d, int i, double d2, int i2)
+{
+    return d < d2 || (d == d2 && i < i2);
+}
+
+// Slots [0, KMAX-k) hold a sentinel smaller than any key, so the k real
+// entries always end at slot KMAX-1 and every register index is static.
+template <int KMAX>
+__global__ __launch_bounds__(256) void knn_query_kernel(GridParams g, const int *__restrict__ cell_start,
+                                                        const double *__restrict__ cell_xy,
+                                                        const int *__restrict__ cell_idx,
+                                                        const int *__restrict__ cell_grp,
+                                                        const double *__restrict__ Q,
+                                                        const int *__restrict__ qgroup,
+                                                        const int *__restrict__ qorder, long M, int k,
+                                                        int *__restrict__ idx_out, double *__restrict__ d2_out)
+{
+#pragma clang fp contract(off)
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    const long q = qorder ? (long)qorder[t] : t;
+    const double qx = Q[2 * q], qy = Q[2 * q + 1];
+    const int qg = (cell_grp && qgroup) ? qgroup[q] : -1;
+
+    double kd[KMAX];
+    int ki[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        bool sentinel = j < KMAX - k;
+        kd[j] = sentinel ? -1.0 : INFINITY;
+        ki[j] = -1;
+    }
+
+    const double fx = (qx - g.x0) / g.csx, fy = (qy - g.y0) / g.csy;
+    // clamp far-away queries into a range where int arithmetic is safe
+    const int qcx = (int)fmin(fmax(floor(fx), -2.0), (double)g.gx + 1.0);
+    const int qcy = (int)fmin(fmax(floor(fy), -2.0), (double)g.gy + 1.0);
+    const int rlimit = g.gx + g.gy + 4;
+
+    for (int r = 0; r <= rlimit; ++r) {
+        const int ncell = r == 0 ? 1 : 8 * r;
+        for (int tc = 0; tc < ncell; ++tc) {
+            int dxc = 0, dyc = 0;
+            if (r > 0) {
+                int side = tc / (2 * r), off = tc - side * 2 * r;
+                if (side == 0) { dxc = -r + off; dyc = -r; }
+                else if (side == 1) { dxc = r; dyc = -r + off; }
+                else if (side == 2) { dxc = r - off; dyc = r; }
+                else { dxc = -r; dyc = r - off; }
+            }
+            const int cx = qcx + dxc, cy = qcy + dyc;
+            if (cx < 0 || cx >= g.gx || cy < 0 || cy >= g.gy) continue;
+            const double rx0 = g.x0 + cx * g.csx, rx1 = rx0 + g.csx;
+            const double ry0 = g.y0 + cy * g.csy, ry1 = ry0 + g.csy;
+            const double ddx = qx < rx0 ? rx0 - qx : (qx > rx1 ? qx - rx1 : 0.0);
+            const double ddy = qy < ry0 ? ry0 - qy : (qy > ry1 ? qy - ry1 : 0.0);
+            const double md = (ddx * ddx + ddy * ddy) * (1.0 - 1e-9);
+            if (md > kd[KMAX - 1]) continue;
+            const int c = cy * g.gx + cx;
+            const int p0 = cell_start[c], p1 = cell_start[c + 1];
+            for (int p = p0; p < p1; ++p) {
+                const double px = cell_xy[2 * (long)p], py = cell_xy[2 * (long)p + 1];
+                const int id = cell_idx[p];
+                if (qg >= 0) {
+                    const int h = cell_grp[p];
+                    if (h != 0 && h != qg) continue;
+                }
+                const double dx = px - qx, dy = py - qy;
+                const double dd = dx * dx;
+                const double ee = dy * dy;
+                const double d = dd + ee;
+                if (key_less(d, id, kd[KMAX - 1], ki[KMAX - 1])) {
+                    // sorted insertion; the last entry drops out
+                    bool lt_next = true;
+#pragma unroll
+                    for (int j = KMAX - 1; j >= 1; --j) {
+                        const bool lt = key_less(d, id, kd[j - 1], ki[j - 1]);
+                        const double nd = lt ? kd[j - 1] : (lt_next ? d : kd[j]);
+                        const int ni = lt ? ki[j - 1] : (lt_next ? id : ki[j]);
+                        kd[j] = nd;
+                        ki[j] = ni;
+                        lt_next = lt;
+                    }
+                    if (lt_next) { kd[0] = d; ki[0] = id; }
+                }
+            }
+        }
+        // every unvisited cell lies outside the block of rings <= r
+        const double bx0 = qx - (g.x0 + (double)(qcx - r) * g.csx);
+        const double bx1 = (g.x0 + (double)(qcx + r + 1) * g.csx) - qx;
+        const double by0 = qy - (g.y0 + (double)(qcy - r) * g.csy);
+        const double by1 = (g.y0 + (double)(qcy + r + 1) * g.csy) - qy;
+        const double b = fmin(fmin(bx0, bx1), fmin(by0, by1));
+        if (b > 0.0 && b * b * (1.0 - 1e-9) > kd[KMAX - 1]) break;
+        if (qcx - r <= 0 && qcy - r <= 0 && qcx + r >= g.gx - 1 && qcy + r >= g.gy - 1) break;
+    }
+
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        if (j >= KMAX - k) {
+            idx_out[q * k + (j - (KMAX - k))] = ki[j];
+            if (d2_out) d2_out[q * k + (j - (KMAX - k))] = kd[j];
+        }
+    }
+}
+
+template <int KMAX>
+static void knn_launch_t(GridParams g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
+                         const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
+                         double *d2, hipStream_t s)
+{
+    knn_query_kernel<KMAX><<<(unsigned)((M + 255) / 256), 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup,
+                                                                         qorder, M, k, idx, d2);
+}
+
+void launch_knn_query(GridParams g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
+                      const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
+                      double *d2, hipStream_t s)
+{
+    if (M <= 0) return;
+    if (k <= 1) knn_launch_t<1>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else if (k <= 8) knn_launch_t<8>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else if (k <= 16) knn_launch_t<16>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else if (k <= 24) knn_launch_t<24>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else if (k <= 32) knn_launch_t<32>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else if (k <= 48) knn_launch_t<48>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else if (k <= 64) knn_launch_t<64>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else if (k <= 96) knn_launch_t<96>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else if (k <= 128) knn_launch_t<128>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    else throw Error{2, "knn: k > 128 is not supported"};
+}
+
+}  // namespace rbffd

@@ -1,0 +1,390 @@
+// stencil_impl.hpp -- K2: batched assembly and solve of the local RBF-FD systems.
+//
+// Replaces hot loops 1 and 2 of the reference (src/generate_operator.jl:55-65
+// and :89-166, method 3 at :388-467; src/hyperviscosity_operator.jl:66-161).
+// One wavefront owns one output row (a Y node and the stencil of its nearest
+// X node).  Lane i holds row i of the saddle-point system
+//     A = [Phi P; P^T 0],  Phi_ij = |xi_i - xi_j|^m,  P_iq = x^a y^b (xi_i)
+// and the matching right-hand-side entries, all in registers.  The system is
+// solved by Gauss-Jordan elimination with partial pivoting: a DPP max
+// reduction picks the pivot row, the pivot lane publishes its row through LDS
+// as a broadcast, and every other lane eliminates its own row.  Rows are never
+// moved.  In SIMD, eliminating above the pivot costs nothing extra (those lanes
+// would idle otherwise), and it removes the serial back substitution.
+//
+// precision 0: FP64 throughout (the plain FP64 solver).
+// precision 1: FP32 Gauss-Jordan on [A | I | B] gives an FP32 inverse and a
+//              first solution; FP64 refinement x += inv (b - A x) with A and b
+//              in FP64 runs until the update is below tol (status 3 if not).
+//
+// Weights are scaled as the reference does and written straight into CSR rows
+// whose columns are sorted (rank of each stencil index inside the stencil).
+#pragma once
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace rbffd {
+
+template <int A_, int B_>
+struct cmax { static constexpr int v = A_ > B_ ? A_ : B_; };
+
+__device__ __forceinline__ double recip_fast(double p)
+{
+    double r = __builtin_amdgcn_rcp(p);
+    double e = __builtin_fma(-p, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-p, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ float recip_fast(float p) { return __builtin_amdgcn_rcpf(p); }
+
+// NN: stencil size, NP: monomials, R: right-hand sides, MIXED: precision 1.
+template <int NN, int NP, int R, bool MIXED, int WPB>
+struct SolveShape {
+    static constexpr int NS = NN + NP;
+    static constexpr int ROWS = (NS + 63) / 64;
+    static constexpr int AUG = MIXED ? NS : 0;        // identity columns
+    static constexpr int WIDTH = NS + AUG + R;        // augmented row
+    static constexpr int BUF = cmax<NN * NP, 2 * NS * R>::v;
+};
+
+template <typename TF, int NN, int NP, int R, bool MIXED, int WPB>
+struct alignas(16) WaveLDS {
+    using S = SolveShape<NN, NP, R, MIXED, WPB>;
+    double xi[NN][2];
+    double buf[S::BUF];
+    TF piv[S::WIDTH + 4];
+    int g[NN];
+};
+
+template <typename TF, int NN, int NP, int R, bool MIXED, int WPB>
+__global__ __launch_bounds__(64 * WPB) void solve_kernel(const SolveArgs a)
+{
+    using S = SolveShape<NN, NP, R, MIXED, WPB>;
+    constexpr int NS = S::NS, ROWS = S::ROWS, WIDTH = S::WIDTH, AUG = S::AUG;
+    static_assert(NN <= 64, "stencil size above 64 needs another lane mapping");
+    static_assert(NS <= 128, "system size above 128 needs another lane mapping");
+    static_assert(!MIXED || ROWS == 1, "mixed precision keeps one row per lane");
+    __shared__ WaveLDS<TF, NN, NP, R, MIXED, WPB> lds_all[WPB];
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const long task = (long)blockIdx.x * WPB + w;
+    if (task >= a.ntasks) return;  // whole wavefront; no block barriers below
+    auto &L = lds_all[w];
+
+    const long yk = a.rows ? (long)a.rows[task] : task;
+    const long xs = a.y2x ? (long)a.y2x[yk] : yk;
+    const int gidx = lane < NN ? a.stencils[xs * NN + lane] : 0;
+    const int c = __builtin_amdgcn_readfirstlane(gidx);
+
+    // ---- stencil frame (src/scalestencil.jl:10-21, src/generate_operator.jl:100-134)
+    double sx = 1.0, sy = 1.0, xix, xiy, yx, yy, ux, uy;
+    {
+#pragma clang fp contract(off)
+        const double cx = a.X[2 * (long)c], cy = a.X[2 * (long)c + 1];
+        const double px = lane < NN ? a.X[2 * (long)gidx] : cx;
+        const double py = lane < NN ? a.X[2 * (long)gidx + 1] : cy;
+        const double dx = px - cx, dy = py - cy;
+        if (a.method == 3) {
+            // src/generate_operator.jl:393-410: shift only, centre := (eps, eps);
+            // RBF RHS at +xi (:433), polynomial RHS at the centre (:429-430)
+            xix = lane == 0 ? kEps : dx;
+            xiy = lane == 0 ? kEps : dy;
+            yx = kEps; yy = kEps;
+            ux = xix; uy = xiy;
+        } else {
+            const double mx = wave_max_nonneg_f64(lane < NN ? __builtin_fabs(dx) : 0.0);
+            const double my = wave_max_nonneg_f64(lane < NN ? __builtin_fabs(dy) : 0.0);
+            sx = 1.0 / mx;
+            sy = 1.0 / my;
+            xix = dx * sx;
+            xiy = dy * sy;
+            yx = (a.Y[2 * yk] - cx) * sx;
+            yy = (a.Y[2 * yk + 1] - cy) * sy;
+            ux = yx - xix;
+            uy = yy - xiy;
+            if (ux == 0.0) ux = kEps;
+            if (uy == 0.0) uy = kEps;
+        }
+    }
+
+    // ---- publish xi and the monomial block for the transposed rows
+    if (lane < NN) {
+        L.xi[lane][0] = xix;
+        L.xi[lane][1] = xiy;
+        L.g[lane] = gidx;
+        constexpr int DEG = mono_deg(NP - 1);
+        double pxp[DEG + 1], pyp[DEG + 1];
+        pxp[0] = 1.0; pyp[0] = 1.0;
+#pragma unroll
+        for (int d = 1; d <= DEG; ++d) { pxp[d] = pxp[d - 1] * xix; pyp[d] = pyp[d - 1] * xiy; }
+#pragma unroll
+        for (int q = 0; q < NP; ++q) L.buf[lane * NP + q] = pxp[mono_ea(q)] * pyp[mono_eb(q)];
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- assemble rows (src/interpolationmatrix.jl:2-5) and right-hand sides (:157)
+    TF row[ROWS][WIDTH];
+    double A64[MIXED ? NS : 1];
+    double b64[MIXED ? R : 1];
+    const int m = a.m;
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr) {
+        const int i = lane + 64 * rr;
+        if (i < NN) {
+#pragma unroll
+            for (int j = 0; j < NN; ++j) {
+                const double ddx = xix - L.xi[j][0], ddy = xiy - L.xi[j][1];
+                const double v = phs_value(m, ddx * ddx + ddy * ddy);
+                row[rr][j] = (TF)v;
+                if constexpr (MIXED) A64[j] = v;
+            }
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const double v = L.buf[i * NP + q];
+                row[rr][NN + q] = (TF)v;
+                if constexpr (MIXED) A64[NN + q] = v;
+            }
+        } else if (i < NS) {
+            const int q = i - NN;
+#pragma unroll
+            for (int j = 0; j < NN; ++j) {
+                const double v = L.buf[j * NP + q];
+                row[rr][j] = (TF)v;
+                if constexpr (MIXED) A64[j] = v;
+            }
+#pragma unroll
+            for (int j = NN; j < NS; ++j) {
+                row[rr][j] = (TF)0;
+                if constexpr (MIXED) A64[j] = 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) row[rr][j] = (TF)0;
+        }
+        if constexpr (MIXED) {
+#pragma unroll
+            for (int j = 0; j < AUG; ++j) row[rr][NS + j] = (TF)(j == i ? 1 : 0);
+        }
+        // right-hand sides: RBF rows at u (eps-substituted), polynomial rows at
+        // the shifted and scaled Y point (src/generate_operator.jl:68-79)
+        const int q = i - NN;
+        int ea = 0, eb = 0;
+        if (i >= NN && i < NS) {
+            int t = 0;
+            while ((t + 1) * (t + 2) / 2 <= q) ++t;
+            ea = q - t * (t + 1) / 2;
+            eb = t - ea;
+        }
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) {
+            double v = 0.0;
+            if (cc < a.nops) {
+                const int op = a.op_kind[cc], kk = a.op_k[cc];
+                if (i < NN) v = op_phs(op, kk, m, ux, uy, sx, sy);
+                else if (i < NS) v = op_mono(op, kk, ea, eb, yx, yy, sx, sy);
+            }
+            row[rr][NS + AUG + cc] = (TF)v;
+            if constexpr (MIXED) b64[cc] = v;
+        }
+    }
+
+    // ---- Gauss-Jordan with partial pivoting
+    uint32_t done = 0;
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr)
+        if (lane + 64 * rr >= NS) done |= 1u << rr;
+    int pstep[ROWS];
+    TF pval[ROWS];
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr) { pstep[rr] = 0; pval[rr] = (TF)1; }
+    bool singular = false;
+
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        uint32_t key = 0;
+#pragma unroll
+        for (int rr = 0; rr < ROWS; ++rr) {
+            if (!((done >> rr) & 1u)) {
+                const float f = __builtin_fabsf((float)row[rr][k]);
+                const uint32_t kr = (__float_as_uint(f) & 0xFFFFFF00u) | 0x80u | (uint32_t)(127 - (lane + 64 * rr));
+                key = key > kr ? key : kr;
+            }
+        }
+        const uint32_t kmax = wave_max_u32(key);
+        const int v = 127 - (int)(kmax & 127u);
+        const int plane = v & 63, psel = v >> 6;
+        const bool iam = lane == plane;
+        if (iam) {
+#pragma unroll
+            for (int j = k; j < WIDTH; ++j) {
+                TF e = row[0][j];
+                if constexpr (ROWS > 1) e = psel ? row[1][j] : e;
+                L.piv[j] = e;
+            }
+#pragma unroll
+            for (int rr = 0; rr < ROWS; ++rr)
+                if (psel == rr) { pstep[rr] = k; pval[rr] = row[rr][k]; done |= 1u << rr; }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const TF p = L.piv[k];
+        singular |= (p == (TF)0);
+        const TF rp = recip_fast(p);
+#pragma unroll
+        for (int rr = 0; rr < ROWS; ++rr) {
+            const bool active = (lane + 64 * rr < NS) && !(iam && psel == rr);
+            if (active) {
+                const TF l = row[rr][k] * rp;
+#pragma unroll
+                for (int j = k + 1; j < WIDTH; ++j) row[rr][j] = __builtin_fma(-l, L.piv[j], row[rr][j]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // ---- solution: x_{pstep} = (augmented rhs) / pivot
+    double xsol[ROWS][R];
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr) {
+        const double d = (double)pval[rr];
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) xsol[rr][cc] = (double)row[rr][NS + AUG + cc] / d;
+    }
+
+    bool converged = true;
+    if constexpr (MIXED) {
+        // inverse row of the pivot step, FP64 copy of inv = row[NS..2NS) / pivot
+        double invr[NS];
+        const double rpv = 1.0 / (double)pval[0];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) invr[j] = (double)row[0][NS + j] * rpv;
+        double *xb = L.buf;            // NS x R: x by unknown
+        double *rb = L.buf + NS * R;   // NS x R: residual by equation
+        const bool real_row = lane < NS;
+        converged = false;
+        for (int it = 0; it < a.max_iter; ++it) {
+            if (real_row) {
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) xb[pstep[0] * R + cc] = xsol[0][cc];
+            }
+            __builtin_amdgcn_wave_barrier();
+            double res[R];
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) res[cc] = b64[cc];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) res[cc] = __builtin_fma(-A64[j], xb[j * R + cc], res[cc]);
+            }
+            if (real_row) {
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) rb[lane * R + cc] = res[cc];
+            }
+            __builtin_amdgcn_wave_barrier();
+            double dmax = 0.0, xmax = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) {
+                double dcc = 0.0;
+#pragma unroll
+                for (int j = 0; j < NS; ++j) dcc = __builtin_fma(invr[j], rb[j * R + cc], dcc);
+                if (real_row) {
+                    xsol[0][cc] += dcc;
+                    // weights only: rows >= NN are Lagrange multipliers
+                    if (pstep[0] < NN) {
+                        dmax = fmax(dmax, __builtin_fabs(dcc));
+                        xmax = fmax(xmax, __builtin_fabs(xsol[0][cc]));
+                    }
+                }
+            }
+            const double gd = wave_max_nonneg_f64(dmax);
+            const double gx = wave_max_nonneg_f64(xmax);
+            __builtin_amdgcn_wave_barrier();
+            if (gd <= a.tol * gx) { converged = true; break; }
+        }
+    }
+
+    // ---- move x to the lane of its stencil node
+    double *xb = L.buf;
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr) {
+        if (lane + 64 * rr < NS) {
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) xb[pstep[rr] * R + cc] = xsol[rr][cc];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    uint32_t bad = 0;
+    if (lane < NN) {
+        int rank = 0;
+#pragma unroll
+        for (int j = 0; j < NN; ++j) rank += L.g[j] < gidx ? 1 : 0;
+        const long base = yk * NN + rank;
+        if (a.colidx) a.colidx[base] = gidx;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) {
+            if (cc < a.nops) {
+                const double f = a.method == 3 ? 1.0 : op_post_scale(a.op_kind[cc], a.op_k[cc], sx, sy);
+                const double wv = f * xb[lane * R + cc];
+                if (!__builtin_isfinite(wv)) bad = 1;
+                a.vals[cc][base] = wv;
+            }
+        }
+    }
+    bad = wave_or_u32(bad);
+    if (a.status && lane == 0) {
+        int st = 0;
+        if (singular) st = 1;
+        else if (bad) st = 2;
+        else if (!converged) st = 3;
+        if (a.status_accumulate) a.status[yk] = max(a.status[yk], st);
+        else a.status[yk] = st;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+constexpr int kWPB = 4;
+
+template <typename TF, int NN, int NP, int R, bool MIXED>
+void launch_t(const SolveArgs &a, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((a.ntasks + kWPB - 1) / kWPB);
+    solve_kernel<TF, NN, NP, R, MIXED, kWPB><<<blocks, 64 * kWPB, 0, s>>>(a);
+}
+
+template <int NN, int NP, int R>
+void launch_prec(const SolveArgs &a, int precision, hipStream_t s)
+{
+    constexpr int NS = NN + NP;
+    if (precision == 1) {
+        if constexpr (NS <= 64) launch_t<float, NN, NP, R, true>(a, s);
+        else throw Error{2, "mixed precision needs n + P <= 64"};
+    } else {
+        launch_t<double, NN, NP, R, false>(a, s);
+    }
+}
+
+// Right-hand-side widths compiled per shape; more operators run in chunks.
+template <int NN, int NP>
+void launch_shape(const SolveArgs &a, int precision, hipStream_t s)
+{
+    if (a.nops <= 1) launch_prec<NN, NP, 1>(a, precision, s);
+    else if (a.nops <= 2) launch_prec<NN, NP, 2>(a, precision, s);
+    else if (a.nops <= 3) launch_prec<NN, NP, 3>(a, precision, s);
+    else if (a.nops <= kMaxRhs) launch_prec<NN, NP, kMaxRhs>(a, precision, s);
+    else throw Error{2, "at most 6 right-hand sides per launch"};
+}
+
+}  // namespace rbffd
+
+// One translation unit per stencil shape keeps the build parallel.
+#define RBFFD_DEFINE_SHAPE(nn, pp)                                                             \
+    namespace rbffd {                                                                          \
+    void launch_solve_##nn##_##pp(const SolveArgs &a, int precision, hipStream_t s)            \
+    {                                                                                          \
+        launch_shape<nn, pp>(a, precision, s);                                                 \
+    }                                                                                          \
+    }
+
+

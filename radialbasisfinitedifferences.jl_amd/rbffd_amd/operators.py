@@ -1,0 +1,235 @@
+"""Host-side mirror of the operator API of RadialBasisFiniteDifferences.jl,
+running on librbffd (gfx950 HIP kernels behind a C ABI).
+
+    generate_operator(X, Y, p, n, polydeg)                       src/generate_operator.jl:29
+    generate_operator(X, Y, p, n, polydeg, X_idx_in, X_idx_bc,
+                      X_idx_bc_g, Y_idx_in, Y_idx_bc, Y_idx_bc_g)   src/generate_operator.jl:192
+    generate_operator(X, p, n, polydeg)                           src/generate_operator.jl:354
+        -> (E, Dx, Dy, Dxx, Dyy, Dxy)
+    hyperviscosity_operator(k, X, Y, p, n, polydeg[, index sets])  src/hyperviscosity_operator.jl:26,177
+    hyperviscosity_operator(k, X, p, n, polydeg)                   src/hyperviscosity_operator.jl:314
+        -> (Dxk, Dyk)
+
+Arguments keep the reference's meaning.  Points are (N, 2) float64 arrays
+(the memory of Vector{SVector{2,Float64}}); index sets are 0-based here (the
+Julia binding passes 1-based sets with index_base = 1).  Results are
+scipy.sparse.csc_matrix with the layout sparse(I, J, V) gives: rows sorted in
+each column, explicit zeros kept, shape (number of Y nodes, largest referenced
+column + 1).  Errors follow the reference: SingularException for an exactly
+singular local system, UndefRefError for a node in no index set, ValueError
+where Julia raises ArgumentError (e.g. n larger than the node count).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import OP, PREC, check, rbffd_boundary, rbffd_stats
+from .device import Context, DeviceArray
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _pts(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] != 2:
+        raise ValueError("points must be an (N, 2) array")
+    return a
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _Boundary:
+    """Keeps the flattened index sets alive for the duration of a call."""
+
+    def __init__(self, X_idx_in, X_idx_bc, X_idx_bc_g, index_base=0):
+        self.in_idx = np.ascontiguousarray(np.asarray(X_idx_in, dtype=np.int64).ravel())
+        bcs = [np.asarray(b, dtype=np.int64).ravel() for b in X_idx_bc]
+        ghs = [np.asarray(g, dtype=np.int64).ravel() for g in X_idx_bc_g]
+        if len(bcs) != len(ghs):
+            raise ValueError("X_idx_bc and X_idx_bc_g must have one entry per boundary marker")
+        self.bc_ptr = np.zeros(len(bcs) + 1, np.int64)
+        self.bc_ptr[1:] = np.cumsum([len(b) for b in bcs])
+        self.gh_ptr = np.zeros(len(ghs) + 1, np.int64)
+        self.gh_ptr[1:] = np.cumsum([len(g) for g in ghs])
+        self.bc_idx = np.ascontiguousarray(np.concatenate(bcs) if bcs else np.zeros(0, np.int64))
+        self.gh_idx = np.ascontiguousarray(np.concatenate(ghs) if ghs else np.zeros(0, np.int64))
+        self.s = rbffd_boundary(index_base, len(self.in_idx), _vp(self.in_idx), len(bcs), _vp(self.bc_ptr),
+                                _vp(self.bc_idx), _vp(self.gh_ptr), _vp(self.gh_idx))
+
+
+def _op_arrays(ops, k):
+    kinds = np.array([OP[o] for o in ops], np.int32)
+    ks = np.array([k if o in ("Dxk", "Dyk", "Lapk") else 0 for o in ops], np.int32)
+    return kinds, ks
+
+
+def operators(X, Y, p, n, polydeg, ops, k=0, boundary=None, precision="fp64", ctx=None,
+              return_stats=False):
+    """Operators `ops` (names from E, Dx, Dy, Dxx, Dyy, Dxy, Dxk, Dyk, Lap,
+    Lapk) of the reference pipeline; Y None selects method 3, boundary =
+    (X_idx_in, X_idx_bc, X_idx_bc_g) selects method 2.  Returns a tuple of
+    csc_matrix in the order of `ops`."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    X = _pts(X)
+    N = len(X)
+    Yp = None if Y is None else _pts(Y)
+    M = N if Yp is None else len(Yp)
+    n = int(n)
+    kinds, ks = _op_arrays(ops, int(k))
+    nops = len(ops)
+    nnz = M * n
+    colptr = np.empty(N + 1, np.int64)
+    rowval = np.empty(nnz, np.int64)
+    nz = [np.empty(nnz, np.float64) for _ in range(nops)]
+    nzp = (ctypes.c_void_p * nops)(*[_vp(a) for a in nz])
+    ncols = ctypes.c_int64()
+    stats = rbffd_stats()
+    bnd = None
+    if boundary is not None:
+        bnd = _Boundary(*boundary)
+    Y_ptr = None
+    if Yp is not None:
+        Y_ptr = _vp(X) if Y is X else _vp(Yp)
+    code = L.rbffd_generate_operator(ctx.handle, _vp(X), N, Y_ptr, M, int(p), n, int(polydeg),
+                                     ctypes.byref(bnd.s) if bnd else None, nops, _vp(kinds), _vp(ks),
+                                     PREC[precision], 0, _vp(colptr), _vp(rowval),
+                                     ctypes.cast(nzp, ctypes.c_void_p), ctypes.byref(ncols), ctypes.byref(stats))
+    check(code)
+    nc = int(ncols.value)
+    mats = tuple(sp.csc_matrix((nz[o], rowval, colptr[: nc + 1]), shape=(M, nc)) for o in range(nops))
+    if return_stats:
+        return mats, stats.as_dict()
+    return mats
+
+
+_SIX = ("E", "Dx", "Dy", "Dxx", "Dyy", "Dxy")
+
+
+def generate_operator(*args, precision="fp64", ctx=None):
+    """Mirror of generate_operator (src/generate_operator.jl:29,192,354):
+    returns (E, Dx, Dy, Dxx, Dyy, Dxy)."""
+    if len(args) == 4:
+        X, p, n, polydeg = args
+        return operators(X, None, p, n, polydeg, _SIX, precision=precision, ctx=ctx)
+    if len(args) == 5:
+        X, Y, p, n, polydeg = args
+        return operators(X, Y, p, n, polydeg, _SIX, precision=precision, ctx=ctx)
+    if len(args) == 11:
+        X, Y, p, n, polydeg, X_idx_in, X_idx_bc, X_idx_bc_g, _Yin, _Ybc, _Ybcg = args
+        # the Y index sets are accepted and unused, as in the reference
+        # (src/calculateneighbors.jl:1-2)
+        return operators(X, Y, p, n, polydeg, _SIX, boundary=(X_idx_in, X_idx_bc, X_idx_bc_g),
+                         precision=precision, ctx=ctx)
+    raise TypeError("generate_operator takes (X, p, n, polydeg), (X, Y, p, n, polydeg) "
+                    "or (X, Y, p, n, polydeg, X_idx_in, X_idx_bc, X_idx_bc_g, Y_idx_in, Y_idx_bc, Y_idx_bc_g)")
+
+
+def hyperviscosity_operator(k_deriv, *args, precision="fp64", ctx=None):
+    """Mirror of hyperviscosity_operator (src/hyperviscosity_operator.jl:26,177,314):
+    returns (Dxk, Dyk), the k-th pure derivatives scaled by s^k."""
+    ops = ("Dxk", "Dyk")
+    if len(args) == 4:
+        X, p, n, polydeg = args
+        return operators(X, None, p, n, polydeg, ops, k=k_deriv, precision=precision, ctx=ctx)
+    if len(args) == 5:
+        X, Y, p, n, polydeg = args
+        return operators(X, Y, p, n, polydeg, ops, k=k_deriv, precision=precision, ctx=ctx)
+    if len(args) == 11:
+        X, Y, p, n, polydeg, X_idx_in, X_idx_bc, X_idx_bc_g, _Yin, _Ybc, _Ybcg = args
+        return operators(X, Y, p, n, polydeg, ops, k=k_deriv, boundary=(X_idx_in, X_idx_bc, X_idx_bc_g),
+                         precision=precision, ctx=ctx)
+    raise TypeError("hyperviscosity_operator takes (k, X, p, n, polydeg), (k, X, Y, p, n, polydeg) "
+                    "or (k, X, Y, p, n, polydeg, 6 index sets)")
+
+
+class Engine:
+    """Device-resident entry points (inputs and outputs stay in HBM)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.L = _lib.load()
+
+    def knn(self, X: DeviceArray, Q: DeviceArray, k: int, xgroup=None, qgroup=None, want_d2=False):
+        N, M = X.shape[0], Q.shape[0]
+        idx = self.ctx.empty((M, k), np.int32)
+        d2 = self.ctx.empty((M, k), np.float64) if want_d2 else None
+        check(self.L.rbffd_knn_device(self.ctx.handle, X.ptr, N, Q.ptr, M, int(k),
+                                      xgroup.ptr if xgroup is not None else None,
+                                      qgroup.ptr if qgroup is not None else None,
+                                      idx.ptr, d2.ptr if d2 is not None else None))
+        return (idx, d2) if want_d2 else idx
+
+    def weights(self, X, Y, stencils, y2x, p, polydeg, ops, k=0, method=1, precision="fp64",
+                colidx=None, vals=None, status=None):
+        n = stencils.shape[1]
+        M = Y.shape[0]
+        kinds, ks = _op_arrays(ops, k)
+        if colidx is None:
+            colidx = self.ctx.empty((M, n), np.int32)
+        if vals is None:
+            vals = [self.ctx.empty((M, n), np.float64) for _ in ops]
+        if status is None:
+            status = self.ctx.empty((M,), np.int32)
+        vp = (ctypes.c_void_p * len(ops))(*[v.ptr for v in vals])
+        check(self.L.rbffd_weights_device(self.ctx.handle, X.ptr, X.shape[0], Y.ptr, M, stencils.ptr, n,
+                                          y2x.ptr if y2x is not None else None, int(p), int(polydeg), int(method),
+                                          len(ops), _vp(kinds), _vp(ks), PREC[precision], colidx.ptr,
+                                          ctypes.cast(vp, ctypes.c_void_p), status.ptr))
+        return colidx, vals, status
+
+    def generate(self, X, Y, p, n, polydeg, ops, k=0, boundary=None, precision="fp64",
+                 colidx=None, vals=None, stencils_out=None, y2x_out=None):
+        """kNN + weights on device.  Y None: method 3; Y is X: collocated."""
+        N = X.shape[0]
+        M = N if Y is None else Y.shape[0]
+        kinds, ks = _op_arrays(ops, k)
+        if colidx is None:
+            colidx = self.ctx.empty((M, n), np.int32)
+        if vals is None:
+            vals = [self.ctx.empty((M, n), np.float64) for _ in ops]
+        vp = (ctypes.c_void_p * len(ops))(*[v.ptr for v in vals])
+        stats = rbffd_stats()
+        bnd = _Boundary(*boundary) if boundary is not None else None
+        Y_ptr = None if Y is None else Y.ptr
+        code = self.L.rbffd_generate_device(self.ctx.handle, X.ptr, N, Y_ptr, M, int(p), int(n), int(polydeg),
+                                            ctypes.byref(bnd.s) if bnd else None, len(ops), _vp(kinds), _vp(ks),
+                                            PREC[precision], colidx.ptr, ctypes.cast(vp, ctypes.c_void_p),
+                                            stencils_out.ptr if stencils_out is not None else None,
+                                            y2x_out.ptr if y2x_out is not None else None, ctypes.byref(stats))
+        check(code)
+        return colidx, vals, stats.as_dict()
+
+    def csr_to_csc(self, colidx, vals, ncols, index_base=0):
+        M, n = colidx.shape
+        colptr = self.ctx.empty((ncols + 1,), np.int64)
+        rowval = self.ctx.empty((M * n,), np.int64)
+        cvals = [self.ctx.empty((M * n,), np.float64) for _ in vals]
+        vp = (ctypes.c_void_p * max(1, len(vals)))(*[v.ptr for v in vals])
+        cp = (ctypes.c_void_p * max(1, len(vals)))(*[v.ptr for v in cvals])
+        used = ctypes.c_int64()
+        check(self.L.rbffd_csr_to_csc_device(self.ctx.handle, M, n, ncols, colidx.ptr, len(vals),
+                                             ctypes.cast(vp, ctypes.c_void_p), colptr.ptr, rowval.ptr,
+                                             ctypes.cast(cp, ctypes.c_void_p), int(index_base), ctypes.byref(used)))
+        return colptr, rowval, cvals, int(used.value)
+
+    def spmv(self, col, vals, x, y, uniform_n=0, rowptr=None, alpha=1.0, beta=0.0):
+        M = y.shape[0]
+        F = 1 if len(x.shape) == 1 else x.shape[1]
+        vt = 1 if vals.dtype == np.float32 else 0
+        check(self.L.rbffd_spmv_device(self.ctx.handle, M, rowptr.ptr if rowptr is not None else None, col.ptr,
+                                       vals.ptr, vt, F, x.ptr, y.ptr, float(alpha), float(beta), int(uniform_n)))
+        return y
