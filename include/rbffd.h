@@ -55,7 +55,7 @@ extern "C" {
 #define RBFFD_MAX_OPS 10
 
 /* precision of the local solves */
-#define RBFFD_PREC_FP64 0   /* FP64 Gauss-Jordan with partial pivoting */
+#define RBFFD_PREC_FP64 0   /* FP64 LU with partial pivoting */
 #define RBFFD_PREC_MIXED 1  /* FP32 Gauss-Jordan inverse + FP64 refinement, FP64 re-solve where it
                                does not converge */
 
@@ -86,6 +86,7 @@ typedef struct rbffd_stats {
     int64_t nonfinite;     /* rows with non-finite weights */
     int64_t refined_fallback; /* mixed precision: rows re-solved in FP64 */
     double ms_knn, ms_solve, ms_fallback;  /* device time per phase (HIP events) */
+    int64_t lanes_per_stencil; /* layout of the FP64 solve kernel (chosen by the autotuner) */
 } rbffd_stats;
 
 /* ---------------------------------------------------------------- context */

@@ -5,8 +5,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <exception>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "internal.hpp"
@@ -21,6 +24,9 @@ struct rbffd_context {
     struct Buf { void *p = nullptr; size_t bytes = 0; };
     Buf bufs[32];
     hipEvent_t ev[8] = {};
+    hipEvent_t tune_ev[2] = {};
+    // autotuner: (n, P, right-hand sides) -> lanes per stencil of the FP64 kernel
+    std::map<std::tuple<int, int, int>, int> lu_group;
 };
 
 namespace {
@@ -122,14 +128,13 @@ void knn_impl(rbffd_context *c, const double *X, long N, const double *Q, long M
     if (M <= 0) return;
     hipStream_t s = c->stream;
     const int nb = (int)std::min<long>(1024, (N + 255) / 256);
-    double *part = (double *)ws(c, S_BBOX_PART, sizeof(double) * 4 * nb);
-    double *box = (double *)ws(c, S_BBOX, sizeof(double) * 4);
+    double *part = (double *)ws(c, S_BBOX_PART, sizeof(double) * 5 * nb);
+    double *box = (double *)ws(c, S_BBOX, sizeof(double) * 5);
     rbffd::launch_bbox(X, N, box, part, nb, s);
-    double hb[4];
+    double hb[5];
     RBFFD_HIP(hipMemcpyAsync(hb, box, sizeof(hb), hipMemcpyDeviceToHost, s));
     RBFFD_HIP(hipStreamSynchronize(s));
-    if (!(std::isfinite(hb[0]) && std::isfinite(hb[1]) && std::isfinite(hb[2]) && std::isfinite(hb[3])))
-        throw Error{RBFFD_ERR_NONFINITE, "knn: non-finite coordinates"};
+    if (hb[4] != 0.0) throw Error{RBFFD_ERR_NONFINITE, "knn: non-finite coordinates"};
 
     const double w = hb[2] - hb[0], h = hb[3] - hb[1];
     const double ppc = std::max(2.0, k / 6.0);  // points per cell
@@ -192,6 +197,47 @@ void check_ops(int nops, const int *ops, const int *opk)
     }
 }
 
+// Lanes per stencil for the FP64 kernel.  Where two layouts are compiled for a
+// shape, the first call with that shape on a context times both on a sample
+// of the batch (the sample's outputs are rewritten by the real launch) and
+// keeps the faster.  RBFFD_LU_GROUP=<lanes> forces a layout.
+int choose_group(rbffd_context *c, const rbffd::SolveArgs &a)
+{
+    int groups[2];
+    rbffd::solve_groups(a.n, a.P, 0, groups);
+    if (const char *env = std::getenv("RBFFD_LU_GROUP")) {
+        const int g = std::atoi(env);
+        if (g == groups[0] || (g == groups[1] && g != 0)) return g;
+    }
+    if (groups[1] == 0) return groups[0];
+    const int rb = a.nops <= 3 ? a.nops : rbffd::kMaxRhs;
+    auto key = std::make_tuple(a.n, a.P, rb);
+    auto it = c->lu_group.find(key);
+    if (it != c->lu_group.end()) return it->second;
+    if (a.ntasks < 4096) return groups[0];  // too small to time; keep the default
+    for (auto &e : c->tune_ev)
+        if (!e) RBFFD_HIP(hipEventCreate(&e));
+    rbffd::SolveArgs t = a;
+    t.ntasks = std::min<long>(a.ntasks, 1L << 16);
+    t.status_accumulate = 0;
+    float best = 1e30f;
+    int pick = groups[0];
+    for (int rep = 0; rep < 2; ++rep) {  // the first pass warms up both kernels
+        for (int gi = 0; gi < 2; ++gi) {
+            t.group = groups[gi];
+            RBFFD_HIP(hipEventRecord(c->tune_ev[0], c->stream));
+            rbffd::launch_solve(t, 0, c->stream);
+            RBFFD_HIP(hipEventRecord(c->tune_ev[1], c->stream));
+            RBFFD_HIP(hipEventSynchronize(c->tune_ev[1]));
+            float ms = 0;
+            RBFFD_HIP(hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
+            if (rep == 1 && ms < best) { best = ms; pick = groups[gi]; }
+        }
+    }
+    c->lu_group[key] = pick;
+    return pick;
+}
+
 void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, long M, const int *stencils,
                   int n, const int *y2x, const int *rows, long nrows, int p, int polydeg, int method,
                   int nops, const int *ops, const int *opk, int precision, int *colidx,
@@ -233,7 +279,8 @@ void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, lo
         a.status = status;
         a.status_accumulate = o0 > 0;
         a.max_iter = 8;
-        a.tol = 1e-14;
+        a.tol = 4e-16;
+        a.group = precision == RBFFD_PREC_FP64 ? choose_group(c, a) : 0;
         rbffd::launch_solve(a, precision, c->stream);
         RBFFD_HIP(hipGetLastError());
     }
@@ -346,6 +393,11 @@ void generate_impl(rbffd_context *c, const double *X, long N, const double *Y, l
         stats->ms_knn = t01;
         stats->ms_solve = t12;
         stats->ms_fallback = t23;
+        int groups[2];
+        rbffd::solve_groups(n, n_monomials(polydeg), 0, groups);
+        const int rb = std::min(nops, 3) < nops ? rbffd::kMaxRhs : nops;
+        auto it = c->lu_group.find(std::make_tuple(n, n_monomials(polydeg), rb));
+        stats->lanes_per_stencil = it != c->lu_group.end() ? it->second : groups[0];
     }
     if (hc[1] > 0)
         throw Error{RBFFD_ERR_SINGULAR, std::to_string(hc[1]) + " stencil systems are exactly singular"};
@@ -424,6 +476,8 @@ int rbffd_destroy(rbffd_context *c)
         for (auto &b : c->bufs)
             if (b.p) (void)hipFree(b.p);
         for (auto &e : c->ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &e : c->tune_ev)
             if (e) (void)hipEventDestroy(e);
         if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
         delete c;

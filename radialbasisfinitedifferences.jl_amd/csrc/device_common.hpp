@@ -135,7 +135,7 @@ __device__ __forceinline__ double binom_d(int n, int k)
 // Operator applied to the PHS at v (Laplacian-type operators carry the
 // per-axis scales inside; single-derivative operators are rescaled after the
 // solve, src/generate_operator.jl:161-166).
-__device__ inline double op_phs(int op, int k, int m, double vx, double vy, double sx, double sy)
+__device__ __attribute__((noinline)) double op_phs(int op, int k, int m, double vx, double vy, double sx, double sy)
 {
     switch (op) {
     case OP_E:   return phs_deriv(m, 0, 0, vx, vy);
@@ -159,7 +159,7 @@ __device__ inline double op_phs(int op, int k, int m, double vx, double vy, doub
     }
 }
 
-__device__ inline double op_mono(int op, int k, int e, int f, double x, double y, double sx, double sy)
+__device__ __attribute__((noinline)) double op_mono(int op, int k, int e, int f, double x, double y, double sx, double sy)
 {
     switch (op) {
     case OP_E:   return mono_deriv(e, f, 0, 0, x, y);

@@ -31,7 +31,8 @@ struct GridParams {
     int gx, gy;       // cells per axis
 };
 
-// Bounding box of X (device), result in box[4] = (xmin, ymin, xmax, ymax).
+// Bounding box of X (device): box[5] = (xmin, ymin, xmax, ymax, non-finite count);
+// partials holds 5 values per block.
 void launch_bbox(const double *X, long N, double *box, double *partials, int nblocks, hipStream_t s);
 // Cell index of every point and the per-cell counts (counts must be zeroed).
 void launch_cell_count(const double *X, long N, GridParams g, int *cell_of, int *counts, hipStream_t s);
@@ -65,6 +66,7 @@ struct SolveArgs {
     // mixed precision refinement
     int max_iter;
     double tol;
+    int group;              // lanes per stencil of the FP64 kernel (0: default)
 };
 
 // Which (n, P) shapes have compiled kernels.
@@ -72,11 +74,15 @@ bool solve_supported(int n, int P, int precision);
 // precision: 0 = FP64 Gauss-Jordan, 1 = FP32 Gauss-Jordan inverse + FP64 refinement.
 // a.nops <= kMaxRhs.
 void launch_solve(const SolveArgs &a, int precision, hipStream_t s);
+// Compiled lane-group sizes for a shape: groups[0] default, groups[1] alternative or 0.
+void solve_groups(int n, int P, int precision, int *groups);
 
 // Compiled stencil shapes (n, P): the reference's tests (20,10) and (42,21),
 // and the synthetic node-set shapes (13,6), (21,10), (31,15), (43,21), (57,28).
 #define RBFFD_SHAPES(X) X(13, 6) X(20, 10) X(21, 10) X(31, 15) X(42, 21) X(43, 21) X(57, 28)
-#define RBFFD_DECLARE_SHAPE(nn, pp) void launch_solve_##nn##_##pp(const SolveArgs &a, int precision, hipStream_t s);
+#define RBFFD_DECLARE_SHAPE(nn, pp)                                                     \
+    void launch_solve_##nn##_##pp(const SolveArgs &a, int precision, hipStream_t s); \
+    void groups_##nn##_##pp(int precision, int *groups);
 RBFFD_SHAPES(RBFFD_DECLARE_SHAPE)
 #undef RBFFD_DECLARE_SHAPE
 

@@ -18,15 +18,18 @@
 namespace rbffd {
 
 // ---------------------------------------------------------------- grid build
+// Bounding box and the count of non-finite coordinates, 5 values per block.
 __global__ void bbox_partial_kernel(const double *X, long N, double *partials)
 {
-    __shared__ double sm[4][256];
-    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY;
+    __shared__ double sm[5][256];
+    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY, bad = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         double x = X[2 * i], y = X[2 * i + 1];
+        if (!__builtin_isfinite(x) || !__builtin_isfinite(y)) bad += 1.0;
         a = fmin(a, x); b = fmin(b, y); c = fmax(c, x); d = fmax(d, y);
     }
     sm[0][threadIdx.x] = a; sm[1][threadIdx.x] = b; sm[2][threadIdx.x] = c; sm[3][threadIdx.x] = d;
+    sm[4][threadIdx.x] = bad;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
@@ -34,21 +37,23 @@ __global__ void bbox_partial_kernel(const double *X, long N, double *partials)
             sm[1][threadIdx.x] = fmin(sm[1][threadIdx.x], sm[1][threadIdx.x + o]);
             sm[2][threadIdx.x] = fmax(sm[2][threadIdx.x], sm[2][threadIdx.x + o]);
             sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + o]);
+            sm[4][threadIdx.x] += sm[4][threadIdx.x + o];
         }
         __syncthreads();
     }
-    if (threadIdx.x < 4) partials[blockIdx.x * 4 + threadIdx.x] = sm[threadIdx.x][0];
+    if (threadIdx.x < 5) partials[blockIdx.x * 5 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
 __global__ void bbox_final_kernel(const double *partials, int nblocks, double *box)
 {
     if (threadIdx.x != 0) return;
-    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY;
+    double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY, bad = 0.0;
     for (int i = 0; i < nblocks; ++i) {
-        a = fmin(a, partials[4 * i]); b = fmin(b, partials[4 * i + 1]);
-        c = fmax(c, partials[4 * i + 2]); d = fmax(d, partials[4 * i + 3]);
+        a = fmin(a, partials[5 * i]); b = fmin(b, partials[5 * i + 1]);
+        c = fmax(c, partials[5 * i + 2]); d = fmax(d, partials[5 * i + 3]);
+        bad += partials[5 * i + 4];
     }
-    box[0] = a; box[1] = b; box[2] = c; box[3] = d;
+    box[0] = a; box[1] = b; box[2] = c; box[3] = d; box[4] = bad;
 }
 
 void launch_bbox(const double *X, long N, double *box, double *partials, int nblocks, hipStream_t s)

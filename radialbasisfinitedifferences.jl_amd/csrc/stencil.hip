@@ -26,4 +26,13 @@ void launch_solve(const SolveArgs &a, int precision, hipStream_t s)
                        std::to_string(a.P) + " monomials"};
 }
 
+void solve_groups(int n, int P, int precision, int *groups)
+{
+    groups[0] = groups[1] = 0;
+#define RBFFD_CASE(nn, pp) \
+    if (n == nn && P == pp) { groups_##nn##_##pp(precision, groups); return; }
+    RBFFD_SHAPES(RBFFD_CASE)
+#undef RBFFD_CASE
+}
+
 }  // namespace rbffd

@@ -1,29 +1,29 @@
-// stencil_impl.hpp -- K2: batched assembly and solve of the local RBF-FD systems.
+// stencil_impl.hpp -- K2, mixed-precision path and the per-shape launchers.
 //
 // Replaces hot loops 1 and 2 of the reference (src/generate_operator.jl:55-65
-// and :89-166, method 3 at :388-467; src/hyperviscosity_operator.jl:66-161).
-// One wavefront owns one output row (a Y node and the stencil of its nearest
-// X node).  Lane i holds row i of the saddle-point system
-//     A = [Phi P; P^T 0],  Phi_ij = |xi_i - xi_j|^m,  P_iq = x^a y^b (xi_i)
-// and the matching right-hand-side entries, all in registers.  The system is
-// solved by Gauss-Jordan elimination with partial pivoting: a DPP max
-// reduction picks the pivot row, the pivot lane publishes its row through LDS
-// as a broadcast, and every other lane eliminates its own row.  Rows are never
-// moved.  In SIMD, eliminating above the pivot costs nothing extra (those lanes
-// would idle otherwise), and it removes the serial back substitution.
-//
-// precision 0: FP64 throughout (the plain FP64 solver).
-// precision 1: FP32 Gauss-Jordan on [A | I | B] gives an FP32 inverse and a
-//              first solution; FP64 refinement x += inv (b - A x) with A and b
-//              in FP64 runs until the update is below tol (status 3 if not).
+// and :89-166, method 3 at :388-467; src/hyperviscosity_operator.jl:66-161)
+// when FP32 factorisation with FP64 refinement is requested.
+// One wavefront owns one output row; lane i holds row i of the saddle-point
+// system in FP32 together with the FP64 row of A and the FP64 right-hand side.
+// FP32 Gauss-Jordan with partial pivoting on [A | I | B] (pivot row broadcast
+// through LDS) gives an FP32 inverse and a first solution; FP64 refinement
+// x += inv (b - A x) then runs until the update reaches the FP64 floor or stops
+// shrinking.  Rows whose last update is above kAcceptRatio get status 3 and
+// are re-solved by the FP64 LU kernel (stencil_lu.hpp).
 //
 // Weights are scaled as the reference does and written straight into CSR rows
 // whose columns are sorted (rank of each stencil index inside the stencil).
 #pragma once
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "stencil_lu.hpp"
+
 
 namespace rbffd {
+
+// refinement: accept a row whose last update is below this, relative to the
+// operator's largest weight
+constexpr double kAcceptRatio = 1e-11;
 
 template <int A_, int B_>
 struct cmax { static constexpr int v = A_ > B_ ? A_ : B_; };
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(64 * WPB) void solve_kernel(const SolveArgs a)
 #pragma unroll
             for (int j = 0; j < NN; ++j) {
                 const double ddx = xix - L.xi[j][0], ddy = xiy - L.xi[j][1];
-                const double v = phs_value(m, ddx * ddx + ddy * ddy);
+                const double v = phs_r2(m, ddx * ddx + ddy * ddy);
                 row[rr][j] = (TF)v;
                 if constexpr (MIXED) A64[j] = v;
             }
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(64 * WPB) void solve_kernel(const SolveArgs a)
             double v = 0.0;
             if (cc < a.nops) {
                 const int op = a.op_kind[cc], kk = a.op_k[cc];
-                if (i < NN) v = op_phs(op, kk, m, ux, uy, sx, sy);
+                if (i < NN) v = op_phs_fast(op, kk, m, ux, uy, sx, sy);
                 else if (i < NS) v = op_mono(op, kk, ea, eb, yx, yy, sx, sy);
             }
             row[rr][NS + AUG + cc] = (TF)v;
@@ -263,6 +263,7 @@ __global__ __launch_bounds__(64 * WPB) void solve_kernel(const SolveArgs a)
         double *rb = L.buf + NS * R;   // NS x R: residual by equation
         const bool real_row = lane < NS;
         converged = false;
+        double prev_ratio = 1e300;
         for (int it = 0; it < a.max_iter; ++it) {
             if (real_row) {
 #pragma unroll
@@ -282,25 +283,33 @@ __global__ __launch_bounds__(64 * WPB) void solve_kernel(const SolveArgs a)
                 for (int cc = 0; cc < R; ++cc) rb[lane * R + cc] = res[cc];
             }
             __builtin_amdgcn_wave_barrier();
-            double dmax = 0.0, xmax = 0.0;
+            // largest update relative to the largest weight, per operator
+            double ratio = 0.0;
 #pragma unroll
             for (int cc = 0; cc < R; ++cc) {
                 double dcc = 0.0;
 #pragma unroll
                 for (int j = 0; j < NS; ++j) dcc = __builtin_fma(invr[j], rb[j * R + cc], dcc);
+                double dm = 0.0, xm = 0.0;
                 if (real_row) {
                     xsol[0][cc] += dcc;
                     // weights only: rows >= NN are Lagrange multipliers
                     if (pstep[0] < NN) {
-                        dmax = fmax(dmax, __builtin_fabs(dcc));
-                        xmax = fmax(xmax, __builtin_fabs(xsol[0][cc]));
+                        dm = __builtin_fabs(dcc);
+                        xm = __builtin_fabs(xsol[0][cc]);
                     }
                 }
+                const double gd = wave_max_nonneg_f64(dm);
+                const double gx = wave_max_nonneg_f64(xm);
+                ratio = fmax(ratio, gx > 0.0 ? gd / gx : 0.0);
             }
-            const double gd = wave_max_nonneg_f64(dmax);
-            const double gx = wave_max_nonneg_f64(xmax);
             __builtin_amdgcn_wave_barrier();
-            if (gd <= a.tol * gx) { converged = true; break; }
+            // stop at the FP64 floor, or when the updates stop shrinking; the
+            // error left is about the size of the last update
+            if (ratio <= a.tol) { converged = true; break; }
+            if (ratio > 0.5 * prev_ratio) { converged = ratio <= kAcceptRatio; break; }
+            prev_ratio = ratio;
+            if (it == a.max_iter - 1) converged = ratio <= kAcceptRatio;
         }
     }
 
@@ -353,6 +362,26 @@ void launch_t(const SolveArgs &a, hipStream_t s)
     solve_kernel<TF, NN, NP, R, MIXED, kWPB><<<blocks, 64 * kWPB, 0, s>>>(a);
 }
 
+// FP64 LU kernel: G lanes per stencil; wavefronts per block chosen so a block
+// stays within 64 KiB of LDS.
+template <int NN, int NP, int R, int G>
+void launch_lu(const SolveArgs &a, hipStream_t s)
+{
+    constexpr int bytes = (int)sizeof(LuLDS<NN, NP, R, G>);
+    constexpr int wpb = bytes * 4 <= 65536 ? 4 : (bytes * 2 <= 65536 ? 2 : 1);
+    constexpr int per_block = wpb * (64 / G);
+    const unsigned blocks = (unsigned)((a.ntasks + per_block - 1) / per_block);
+    solve_lu_kernel<NN, NP, R, G, wpb><<<blocks, 64 * wpb, 0, s>>>(a);
+}
+
+// Lane-group sizes compiled for a system size: the first is the default, a
+// second one is a candidate for the autotuner.
+template <int NS>
+struct LuGroups {
+    static constexpr int first = NS <= 48 ? 16 : (NS <= 64 ? 32 : 64);
+    static constexpr int second = (NS > 32 && NS <= 48) ? 32 : 0;
+};
+
 template <int NN, int NP, int R>
 void launch_prec(const SolveArgs &a, int precision, hipStream_t s)
 {
@@ -361,7 +390,11 @@ void launch_prec(const SolveArgs &a, int precision, hipStream_t s)
         if constexpr (NS <= 64) launch_t<float, NN, NP, R, true>(a, s);
         else throw Error{2, "mixed precision needs n + P <= 64"};
     } else {
-        launch_t<double, NN, NP, R, false>(a, s);
+        constexpr int g1 = LuGroups<NS>::first, g2 = LuGroups<NS>::second;
+        if constexpr (g2 != 0) {
+            if (a.group == g2) { launch_lu<NN, NP, R, g2>(a, s); return; }
+        }
+        launch_lu<NN, NP, R, g1>(a, s);
     }
 }
 
@@ -376,6 +409,14 @@ void launch_shape(const SolveArgs &a, int precision, hipStream_t s)
     else throw Error{2, "at most 6 right-hand sides per launch"};
 }
 
+template <int NN, int NP>
+void shape_groups(int precision, int *groups)
+{
+    constexpr int NS = NN + NP;
+    groups[0] = precision == 1 ? 64 : LuGroups<NS>::first;
+    groups[1] = precision == 1 ? 0 : LuGroups<NS>::second;
+}
+
 }  // namespace rbffd
 
 // One translation unit per stencil shape keeps the build parallel.
@@ -385,6 +426,7 @@ void launch_shape(const SolveArgs &a, int precision, hipStream_t s)
     {                                                                                          \
         launch_shape<nn, pp>(a, precision, s);                                                 \
     }                                                                                          \
+    void groups_##nn##_##pp(int precision, int *groups) { shape_groups<nn, pp>(precision, groups); } \
     }
 
 

@@ -16,6 +16,24 @@ def _csc(t):
     return sp.csc_matrix((nz, rv, cp), shape=sh)
 
 
+def _lu_reference(O, X, p, n, polydeg, ops, k):
+    st, _ = O.knn(X, X, n)
+    opl = [(O.OP_NAMES[o], k) for o in ops]
+    W, _, _ = O.weights(X, X, st, None, p, polydeg, opl, solve="lu")
+    rows = np.repeat(np.arange(len(X)), n).reshape(len(X), n)
+    return {name: O.sparse_csc(rows, st, W[:, j, :]) for j, name in enumerate(ops)}
+
+
+def _rowerr(A, B):
+    D = (A - B).tocsr()
+    R = B.tocsr()
+    num = np.zeros(A.shape[0])
+    den = np.zeros(A.shape[0])
+    np.maximum.at(num, np.repeat(np.arange(A.shape[0]), np.diff(D.indptr)), np.abs(D.data))
+    np.maximum.at(den, np.repeat(np.arange(A.shape[0]), np.diff(R.indptr)), np.abs(R.data))
+    return float((num / np.maximum(den, 1e-300)).max())
+
+
 def _same(A, ref_tuple, tol=1e-10):
     B = _csc(ref_tuple)
     assert A.shape == B.shape
@@ -78,8 +96,13 @@ def test_hyperviscosity_order6(ctx, O):
     print("hyperviscosity_operator(6, X, X, 9, 57, 6)", flush=True)
     Dxk, Dyk = rbffd_amd.hyperviscosity_operator(6, X, X, 9, 57, 6, ctx=ctx)
     ref = O.generate_operator(X, X, 9, 57, 6, ops=("Dxk", "Dyk"), k=6, quad=True)
-    _same(Dxk, ref["Dxk"])
-    _same(Dyk, ref["Dyk"])
+    lu = _lu_reference(O, X, 9, 57, 6, ("Dxk", "Dyk"), 6)
+    # cond(A) of r^9 / degree-6 stencils reaches 1e8 at the domain edge: the bar
+    # is 1e-10 or 4x the error of a backward-stable FP64 LU solve
+    for A, name in ((Dxk, "Dxk"), (Dyk, "Dyk")):
+        e_lu = _rowerr(_csc(lu[name]), _csc(ref[name]))
+        e = _same(A, ref[name], tol=max(1e-10, 4 * e_lu))
+        print(f"  {name}: {e:.2e} (fp64 LU {e_lu:.2e})", flush=True)
 
 
 def _boundary_domain():
