@@ -9,16 +9,19 @@
 //     A = [Phi P; P^T 0],  Phi_ij = |xi_i - xi_j|^m,  P_iq = x^a y^b (xi_i),
 // each row extended by its right-hand-side entries, all in registers.
 // LU with partial pivoting: a DPP max over the group picks the pivot row, the
-// pivot lane publishes that row in its group's LDS slot, and every lane
-// eliminates its own rows with one LDS broadcast read feeding ROWS FMAs.
-// Rows never move (the permutation stays implicit).  Forward substitution
-// rides along in the right-hand-side columns; back substitution broadcasts
-// each solved unknown through LDS.  LU (rather than Gauss-Jordan) keeps the
-// forward error at the level of a backward-stable FP64 solve.
+// pivot lane publishes that row as row k of a packed U factor in LDS, and
+// every lane eliminates its own rows with one LDS broadcast read feeding ROWS
+// FMAs.  Rows never move (the permutation stays implicit), and a column leaves
+// the registers once it is eliminated.  Forward substitution rides along in
+// the right-hand-side columns; back substitution runs from U in LDS with a
+// DPP broadcast of each solved unknown.  LU (rather than Gauss-Jordan) keeps
+// the forward error at the level of a backward-stable FP64 solve.
 //
 // Weights are scaled as the reference does (src/generate_operator.jl:161-166)
 // and written straight into CSR rows with sorted columns.
 #pragma once
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -26,6 +29,18 @@ namespace rbffd {
 
 template <int A_, int B_>
 struct cmax_ { static constexpr int v = A_ > B_ ? A_ : B_; };
+
+// elimination steps per block of equal static row width
+constexpr int kStepBlock = 8;
+
+template <int B, int NB, class F>
+__device__ __forceinline__ void for_blocks(F &&f)
+{
+    if constexpr (B < NB) {
+        f(std::integral_constant<int, B>{});
+        for_blocks<B + 1, NB>(f);
+    }
+}
 
 // Maximum over each aligned group of G lanes; every lane gets its group's value.
 template <int G>
@@ -46,6 +61,45 @@ __device__ __forceinline__ uint32_t group_max_u32(uint32_t v)
         v = a > b ? a : b;
     }
     return v;
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    int2 p = *reinterpret_cast<int2 *>(&v);
+    p.x = __builtin_amdgcn_update_dpp(0, p.x, CTRL, 0xF, 0xF, false);
+    p.y = __builtin_amdgcn_update_dpp(0, p.y, CTRL, 0xF, 0xF, false);
+    return *reinterpret_cast<double *>(&p);
+}
+
+// Value of lane `src` of each G-lane group, in every lane of the group.  src
+// is a constant after unrolling: 16-lane groups use DPP row_newbcast (each DPP
+// row is one group), wider groups a lane shuffle.
+template <int G>
+__device__ __forceinline__ double group_bcast(double v, int src)
+{
+    if constexpr (G == 16) {
+        switch (src) {
+        case 0: return dpp_f64<0x150>(v);
+        case 1: return dpp_f64<0x151>(v);
+        case 2: return dpp_f64<0x152>(v);
+        case 3: return dpp_f64<0x153>(v);
+        case 4: return dpp_f64<0x154>(v);
+        case 5: return dpp_f64<0x155>(v);
+        case 6: return dpp_f64<0x156>(v);
+        case 7: return dpp_f64<0x157>(v);
+        case 8: return dpp_f64<0x158>(v);
+        case 9: return dpp_f64<0x159>(v);
+        case 10: return dpp_f64<0x15A>(v);
+        case 11: return dpp_f64<0x15B>(v);
+        case 12: return dpp_f64<0x15C>(v);
+        case 13: return dpp_f64<0x15D>(v);
+        case 14: return dpp_f64<0x15E>(v);
+        default: return dpp_f64<0x15F>(v);
+        }
+    } else {
+        return __shfl(v, src, G);
+    }
 }
 
 template <int G>
@@ -117,16 +171,20 @@ struct LuShape {
     static constexpr int WIDTH = NS + R;
     static constexpr int S = 64 / G;               // stencils per wavefront
     static constexpr int NODES = (NN + G - 1) / G; // stencil nodes per lane
-    static constexpr int PSTRIDE = WIDTH + 2;      // doubles per pivot slot (keeps groups off one bank)
-    static constexpr int XSTRIDE = NS * R + 2;
+    // U factor, packed by pivot step: row k holds columns k..WIDTH-1
+    static constexpr int UTOT = NS * WIDTH - NS * (NS - 1) / 2;
+    // +kStepBlock: a step writes up to kStepBlock-1 stale values past its row;
+    // +2 keeps the groups' slots off one LDS bank
+    static constexpr int USTRIDE = cmax_<UTOT, NN * NP>::v + kStepBlock + 2;
+    static constexpr int NBLK = (NS + kStepBlock - 1) / kStepBlock;
+    static constexpr __host__ __device__ int uoff(int k) { return k * WIDTH - k * (k - 1) / 2; }
 };
 
 template <int NN, int NP, int R, int G>
 struct alignas(16) LuLDS {
     using T = LuShape<NN, NP, R, G>;
     double xi[T::S][NN][2];
-    double pb[T::S][cmax_<NN * NP, T::XSTRIDE>::v];  // monomial block, later the solution
-    double piv[T::S][T::PSTRIDE];
+    double u[T::S][T::USTRIDE];  // monomial block during assembly, then the U factor
     int g[T::S][NN];
 };
 
@@ -217,7 +275,7 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
 #pragma unroll
             for (int d = 1; d <= DEG; ++d) { pxp[d] = pxp[d - 1] * dxn[r]; pyp[d] = pyp[d - 1] * dyn[r]; }
 #pragma unroll
-            for (int q = 0; q < NP; ++q) L.pb[grp][i * NP + q] = pxp[mono_ea(q)] * pyp[mono_eb(q)];
+            for (int q = 0; q < NP; ++q) L.u[grp][i * NP + q] = pxp[mono_ea(q)] * pyp[mono_eb(q)];
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -235,7 +293,7 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
                 row[rr][j] = phs_r2(m, ddx * ddx + ddy * ddy);
             }
 #pragma unroll
-            for (int q = 0; q < NP; ++q) row[rr][NN + q] = L.pb[grp][i * NP + q];
+            for (int q = 0; q < NP; ++q) row[rr][NN + q] = L.u[grp][i * NP + q];
             // RBF rows of the right-hand sides at u = Yscaled - xi (exact zeros -> eps)
             double ux, uy;
             {
@@ -256,7 +314,7 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
         } else if (i < NS) {
             const int q = i - NN;
 #pragma unroll
-            for (int j = 0; j < NN; ++j) row[rr][j] = L.pb[grp][j * NP + q];
+            for (int j = 0; j < NN; ++j) row[rr][j] = L.u[grp][j * NP + q];
 #pragma unroll
             for (int j = NN; j < NS; ++j) row[rr][j] = 0.0;
             int tt = 0;
@@ -272,92 +330,108 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
     }
     __builtin_amdgcn_wave_barrier();
 
-    // ---- LU with partial pivoting, forward substitution in the RHS columns
+    // ---- LU with partial pivoting, forward substitution in the RHS columns.
+    // Rows slide one place per step (row[j] <- row[j+1] - l u[j+1]), so the
+    // pivot column is always register 0 and every register index is static.
+    // Steps run as a real loop in blocks of kStepBlock with a static width that
+    // shrinks per block: compact code, at most kStepBlock-1 stale columns of
+    // extra work per step.
     uint32_t done = 0;
-    int pstep[ROWS];
-    double pval[ROWS];
 #pragma unroll
-    for (int rr = 0; rr < ROWS; ++rr) {
+    for (int rr = 0; rr < ROWS; ++rr)
         if (t + G * rr >= NS) done |= 1u << rr;
-        pstep[rr] = NS;
-        pval[rr] = 1.0;
-    }
     bool singular = false;
-    double *piv = L.piv[grp];
+    double *U = L.u[grp];
+    int k = 0;
+    for_blocks<0, T::NBLK>([&](auto bc) {
+        constexpr int b = decltype(bc)::value;
+        constexpr int WB = WIDTH - kStepBlock * b;  // >= WIDTH - k for every k of the block
+        const int kend = NS < kStepBlock * (b + 1) ? NS : kStepBlock * (b + 1);
+#pragma nounroll
+        for (; k < kend; ++k) {
+            uint32_t key = 0;
+#pragma unroll
+            for (int rr = 0; rr < ROWS; ++rr) {
+                if (!((done >> rr) & 1u)) {
+                    const float f = __builtin_fabsf((float)row[rr][0]);
+                    const uint32_t kr = (__float_as_uint(f) & 0xFFFFFF00u) | 0x80u | (uint32_t)(127 - (t + G * rr));
+                    key = key > kr ? key : kr;
+                }
+            }
+            const uint32_t kmax = group_max_u32<G>(key);
+            const int v = 127 - (int)(kmax & 127u);
+            const bool iam = t == v % G;
+            const int psel = v / G;
+            double *piv = U + T::uoff(k);  // piv[j] = U(k, k + j)
+#pragma unroll
+            for (int rr = 0; rr < ROWS; ++rr) {
+                if (iam && psel == rr) {
+#pragma unroll
+                    for (int j = 0; j < WB; ++j) piv[j] = row[rr][j];
+                    done |= 1u << rr;
+                }
+            }
+            // the other lanes read what the pivot lane just wrote
+            __builtin_amdgcn_wave_barrier();
+            const double p = piv[0];
+            singular |= (p == 0.0);
+            double rp = __builtin_amdgcn_rcp(p);
+            rp = __builtin_fma(rp, __builtin_fma(-p, rp, 1.0), rp);
+            rp = __builtin_fma(rp, __builtin_fma(-p, rp, 1.0), rp);
+            double l[ROWS];
+#pragma unroll
+            for (int rr = 0; rr < ROWS; ++rr) l[rr] = ((done >> rr) & 1u) ? 0.0 : row[rr][0] * rp;
+#pragma unroll
+            for (int j = 0; j + 1 < WB; ++j) {
+                const double pj = piv[j + 1];
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = __builtin_fma(-l[rr], pj, row[rr][j + 1]);
+            }
+            // the next step's pivot lane overwrites the stale tail read here
+            __builtin_amdgcn_wave_barrier();
+        }
+    });
+    __builtin_amdgcn_wave_barrier();
 
+    // ---- back substitution from the U factor in LDS.  Lane t owns the
+    // unknowns i = t + G s: it holds y_i, forms x_i = y_i / U_ii when its turn
+    // comes, and a group broadcast of x_k lets every lane update the y_i it
+    // owns (i < k).  The lane that owns x_i is the one that writes node i.
+    double yv[ROWS][R], xv[ROWS][R], rdiag[ROWS];
+    int ubase[ROWS];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        uint32_t key = 0;
+    for (int s2 = 0; s2 < ROWS; ++s2) {
+        const int i = t + G * s2;
+        const int ic = i < NS ? i : NS - 1;
+        ubase[s2] = T::uoff(ic) - ic;  // U(i, j) = U[ubase + j]
+        rdiag[s2] = 1.0 / U[ubase[s2] + ic];
 #pragma unroll
-        for (int rr = 0; rr < ROWS; ++rr) {
-            if (!((done >> rr) & 1u)) {
-                const float f = __builtin_fabsf((float)row[rr][k]);
-                const uint32_t kr = (__float_as_uint(f) & 0xFFFFFF00u) | 0x80u | (uint32_t)(127 - (t + G * rr));
-                key = key > kr ? key : kr;
-            }
+        for (int cc = 0; cc < R; ++cc) {
+            yv[s2][cc] = U[ubase[s2] + NS + cc];
+            xv[s2][cc] = 0.0;
         }
-        const uint32_t kmax = group_max_u32<G>(key);
-        const int v = 127 - (int)(kmax & 127u);
-        const bool iam = t == v % G;
-        const int psel = v / G;
-#pragma unroll
-        for (int rr = 0; rr < ROWS; ++rr) {
-            if (iam && psel == rr) {
-#pragma unroll
-                for (int j = k; j < WIDTH; ++j) piv[j] = row[rr][j];
-                pstep[rr] = k;
-                pval[rr] = row[rr][k];
-                done |= 1u << rr;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        const double p = piv[k];
-        singular |= (p == 0.0);
-        double rp = __builtin_amdgcn_rcp(p);
-        rp = __builtin_fma(rp, __builtin_fma(-p, rp, 1.0), rp);
-        rp = __builtin_fma(rp, __builtin_fma(-p, rp, 1.0), rp);
-        double l[ROWS];
-#pragma unroll
-        for (int rr = 0; rr < ROWS; ++rr) {
-            const bool act = !((done >> rr) & 1u);
-            l[rr] = act ? row[rr][k] * rp : 0.0;
-            // eliminated rows read as zero in column k during back substitution
-            row[rr][k] = act ? 0.0 : row[rr][k];
-        }
-#pragma unroll
-        for (int j = k + 1; j < WIDTH; ++j) {
-            const double pj = piv[j];
-#pragma unroll
-            for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = __builtin_fma(-l[rr], pj, row[rr][j]);
-        }
-        __builtin_amdgcn_wave_barrier();
     }
-
-    // ---- back substitution: x_k from the pivot row of step k, broadcast in LDS
-    double rinv[ROWS];
+#pragma nounroll
+    for (int kk = NS - 1; kk >= 0; --kk) {
+        const int sk = kk / G, tk = kk % G;
+        double uik[ROWS];
 #pragma unroll
-    for (int rr = 0; rr < ROWS; ++rr) rinv[rr] = 1.0 / pval[rr];
-    double *xb = L.pb[grp];  // x_k at xb[k * R + c]
+        for (int s2 = 0; s2 < ROWS; ++s2) {
+            const int i = t + G * s2;
+            uik[s2] = (i < kk) ? U[ubase[s2] + kk] : 0.0;
+        }
 #pragma unroll
-    for (int k = NS - 1; k >= 0; --k) {
+        for (int cc = 0; cc < R; ++cc) {
+            double own = 0.0;
 #pragma unroll
-        for (int rr = 0; rr < ROWS; ++rr) {
-            if (pstep[rr] == k) {
+            for (int s2 = 0; s2 < ROWS; ++s2) own = (s2 == sk) ? yv[s2][cc] * rdiag[s2] : own;
+            const double xk = __shfl(own, tk, G);
 #pragma unroll
-                for (int cc = 0; cc < R; ++cc) xb[k * R + cc] = row[rr][NS + cc] * rinv[rr];
+            for (int s2 = 0; s2 < ROWS; ++s2) {
+                xv[s2][cc] = (s2 == sk && t == tk) ? xk : xv[s2][cc];
+                yv[s2][cc] = __builtin_fma(-uik[s2], xk, yv[s2][cc]);
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        if (k > 0) {
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc) {
-                const double xk = xb[k * R + cc];
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr)
-                    row[rr][NS + cc] = __builtin_fma(-row[rr][k], xk, row[rr][NS + cc]);
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
     }
 
     // ---- CSR output: sorted columns by rank inside the stencil
@@ -375,7 +449,7 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
             for (int cc = 0; cc < R; ++cc) {
                 if (cc < a.nops) {
                     const double f = a.method == 3 ? 1.0 : op_post_scale(a.op_kind[cc], a.op_k[cc], sx, sy);
-                    const double wv = f * xb[i * R + cc];
+                    const double wv = f * xv[r][cc];
                     if (!__builtin_isfinite(wv)) bad = 1;
                     a.vals[cc][base] = wv;
                 }
