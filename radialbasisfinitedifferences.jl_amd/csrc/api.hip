@@ -197,19 +197,22 @@ void check_ops(int nops, const int *ops, const int *opk)
     }
 }
 
-// Lanes per stencil for the FP64 kernel.  Where two layouts are compiled for a
-// shape, the first call with that shape on a context times both on a sample
-// of the batch (the sample's outputs are rewritten by the real launch) and
-// keeps the faster.  RBFFD_LU_GROUP=<lanes> forces a layout.
+// Lanes per stencil for the FP64 kernel.  Where several layouts are compiled
+// for a shape, the first call with that shape on a context times each on a
+// sample of the batch (the sample's outputs are rewritten by the real launch)
+// and keeps the fastest.  RBFFD_LU_GROUP=<lanes> forces a layout.
 int choose_group(rbffd_context *c, const rbffd::SolveArgs &a)
 {
-    int groups[2];
+    int groups[rbffd::kMaxGroups];
     rbffd::solve_groups(a.n, a.P, 0, groups);
+    int ncand = 0;
+    while (ncand < rbffd::kMaxGroups && groups[ncand] != 0) ++ncand;
     if (const char *env = std::getenv("RBFFD_LU_GROUP")) {
         const int g = std::atoi(env);
-        if (g == groups[0] || (g == groups[1] && g != 0)) return g;
+        for (int gi = 0; gi < ncand; ++gi)
+            if (g == groups[gi]) return g;
     }
-    if (groups[1] == 0) return groups[0];
+    if (ncand <= 1) return groups[0];
     const int rb = a.nops <= 3 ? a.nops : rbffd::kMaxRhs;
     auto key = std::make_tuple(a.n, a.P, rb);
     auto it = c->lu_group.find(key);
@@ -222,8 +225,8 @@ int choose_group(rbffd_context *c, const rbffd::SolveArgs &a)
     t.status_accumulate = 0;
     float best = 1e30f;
     int pick = groups[0];
-    for (int rep = 0; rep < 2; ++rep) {  // the first pass warms up both kernels
-        for (int gi = 0; gi < 2; ++gi) {
+    for (int rep = 0; rep < 2; ++rep) {  // the first pass warms up every kernel
+        for (int gi = 0; gi < ncand; ++gi) {
             t.group = groups[gi];
             RBFFD_HIP(hipEventRecord(c->tune_ev[0], c->stream));
             rbffd::launch_solve(t, 0, c->stream);
@@ -393,7 +396,7 @@ void generate_impl(rbffd_context *c, const double *X, long N, const double *Y, l
         stats->ms_knn = t01;
         stats->ms_solve = t12;
         stats->ms_fallback = t23;
-        int groups[2];
+        int groups[rbffd::kMaxGroups];
         rbffd::solve_groups(n, n_monomials(polydeg), 0, groups);
         const int rb = std::min(nops, 3) < nops ? rbffd::kMaxRhs : nops;
         auto it = c->lu_group.find(std::make_tuple(n, n_monomials(polydeg), rb));

@@ -74,7 +74,9 @@ bool solve_supported(int n, int P, int precision);
 // precision: 0 = FP64 Gauss-Jordan, 1 = FP32 Gauss-Jordan inverse + FP64 refinement.
 // a.nops <= kMaxRhs.
 void launch_solve(const SolveArgs &a, int precision, hipStream_t s);
-// Compiled lane-group sizes for a shape: groups[0] default, groups[1] alternative or 0.
+// Compiled lane-group sizes for a shape: groups[0] default, groups[1..kMaxGroups-1]
+// alternatives or 0.
+constexpr int kMaxGroups = 3;
 void solve_groups(int n, int P, int precision, int *groups);
 
 // Compiled stencil shapes (n, P): the reference's tests (20,10) and (42,21),

@@ -28,7 +28,7 @@ void launch_solve(const SolveArgs &a, int precision, hipStream_t s)
 
 void solve_groups(int n, int P, int precision, int *groups)
 {
-    groups[0] = groups[1] = 0;
+    groups[0] = groups[1] = groups[2] = 0;
 #define RBFFD_CASE(nn, pp) \
     if (n == nn && P == pp) { groups_##nn##_##pp(precision, groups); return; }
     RBFFD_SHAPES(RBFFD_CASE)

@@ -362,24 +362,40 @@ void launch_t(const SolveArgs &a, hipStream_t s)
     solve_kernel<TF, NN, NP, R, MIXED, kWPB><<<blocks, 64 * kWPB, 0, s>>>(a);
 }
 
-// FP64 LU kernel: G lanes per stencil; wavefronts per block chosen so a block
-// stays within 64 KiB of LDS.
+// Wavefronts per block for a kernel using `bytes` of LDS per wavefront: the
+// most wavefronts resident in a CU's 160 KiB of LDS, blocks within 64 KiB,
+// larger blocks on ties.
+constexpr int kLdsPerCU = 160 * 1024;
+constexpr int resident_waves(int bytes, int wpb)
+{
+    return bytes * wpb > 65536 ? 0 : (kLdsPerCU / (bytes * wpb)) * wpb;
+}
+constexpr int lu_wpb(int bytes)
+{
+    int best = 1;
+    for (int w = 2; w <= 4; ++w)
+        if (resident_waves(bytes, w) >= resident_waves(bytes, best)) best = w;
+    return best;
+}
+
+// FP64 LU kernel: G lanes per stencil.
 template <int NN, int NP, int R, int G>
 void launch_lu(const SolveArgs &a, hipStream_t s)
 {
     constexpr int bytes = (int)sizeof(LuLDS<NN, NP, R, G>);
-    constexpr int wpb = bytes * 4 <= 65536 ? 4 : (bytes * 2 <= 65536 ? 2 : 1);
+    constexpr int wpb = lu_wpb(bytes);
     constexpr int per_block = wpb * (64 / G);
     const unsigned blocks = (unsigned)((a.ntasks + per_block - 1) / per_block);
-    solve_lu_kernel<NN, NP, R, G, wpb><<<blocks, 64 * wpb, 0, s>>>(a);
+    solve_lu_kernel<NN, NP, R, G, wpb, 1><<<blocks, 64 * wpb, 0, s>>>(a);
 }
 
-// Lane-group sizes compiled for a system size: the first is the default, a
-// second one is a candidate for the autotuner.
+// Lane-group sizes compiled for a system size: g0 is the default, the others
+// (0 = none) are candidates for the autotuner.
 template <int NS>
 struct LuGroups {
-    static constexpr int first = NS <= 48 ? 16 : (NS <= 64 ? 32 : 64);
-    static constexpr int second = (NS > 32 && NS <= 48) ? 32 : 0;
+    static constexpr int g0 = NS <= 32 ? 16 : (NS <= 64 ? 32 : 64);
+    static constexpr int g1 = NS <= 48 ? (NS <= 32 ? 32 : 16) : (NS <= 64 ? 64 : 0);
+    static constexpr int g2 = (NS > 32 && NS <= 48) ? 64 : 0;
 };
 
 template <int NN, int NP, int R>
@@ -390,11 +406,14 @@ void launch_prec(const SolveArgs &a, int precision, hipStream_t s)
         if constexpr (NS <= 64) launch_t<float, NN, NP, R, true>(a, s);
         else throw Error{2, "mixed precision needs n + P <= 64"};
     } else {
-        constexpr int g1 = LuGroups<NS>::first, g2 = LuGroups<NS>::second;
+        constexpr int g0 = LuGroups<NS>::g0, g1 = LuGroups<NS>::g1, g2 = LuGroups<NS>::g2;
+        if constexpr (g1 != 0) {
+            if (a.group == g1) { launch_lu<NN, NP, R, g1>(a, s); return; }
+        }
         if constexpr (g2 != 0) {
             if (a.group == g2) { launch_lu<NN, NP, R, g2>(a, s); return; }
         }
-        launch_lu<NN, NP, R, g1>(a, s);
+        launch_lu<NN, NP, R, g0>(a, s);
     }
 }
 
@@ -413,8 +432,9 @@ template <int NN, int NP>
 void shape_groups(int precision, int *groups)
 {
     constexpr int NS = NN + NP;
-    groups[0] = precision == 1 ? 64 : LuGroups<NS>::first;
-    groups[1] = precision == 1 ? 0 : LuGroups<NS>::second;
+    groups[0] = precision == 1 ? 64 : LuGroups<NS>::g0;
+    groups[1] = precision == 1 ? 0 : LuGroups<NS>::g1;
+    groups[2] = precision == 1 ? 0 : LuGroups<NS>::g2;
 }
 
 }  // namespace rbffd

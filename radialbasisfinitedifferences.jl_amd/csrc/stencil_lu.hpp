@@ -9,12 +9,13 @@
 //     A = [Phi P; P^T 0],  Phi_ij = |xi_i - xi_j|^m,  P_iq = x^a y^b (xi_i),
 // each row extended by its right-hand-side entries, all in registers.
 // LU with partial pivoting: a DPP max over the group picks the pivot row, the
-// pivot lane publishes that row as row k of a packed U factor in LDS, and
-// every lane eliminates its own rows with one LDS broadcast read feeding ROWS
-// FMAs.  Rows never move (the permutation stays implicit), and a column leaves
+// pivot lane publishes that row (and its reciprocal pivot) as row k of a
+// packed U factor in LDS, and every lane eliminates its own rows with one LDS
+// broadcast read feeding ROWS FMAs.  The pivot search for step k+1 runs as
+// soon as column k+1 is updated, overlapping the rest of step k.  Rows never move (the permutation stays implicit), and a column leaves
 // the registers once it is eliminated.  Forward substitution rides along in
-// the right-hand-side columns; back substitution runs from U in LDS with a
-// DPP broadcast of each solved unknown.  LU (rather than Gauss-Jordan) keeps
+// the right-hand-side columns; back substitution runs from U in LDS with an
+// LDS broadcast of each solved unknown.  LU (rather than Gauss-Jordan) keeps
 // the forward error at the level of a backward-stable FP64 solve.
 //
 // Weights are scaled as the reference does (src/generate_operator.jl:161-166)
@@ -33,6 +34,17 @@ struct cmax_ { static constexpr int v = A_ > B_ ? A_ : B_; };
 // elimination steps per block of equal static row width
 constexpr int kStepBlock = 8;
 
+// a * b + c into a register distinct from c's.  The compiler's own lowering
+// of the sliding update (row[j] = fma(., ., row[j+1]) in a loop) is a copy
+// plus the two-address v_fmac; the three-address form needs no copy.  Register
+// operands only, no memory access.
+__device__ __forceinline__ double fma3(double a, double b, double c)
+{
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
 template <int B, int NB, class F>
 __device__ __forceinline__ void for_blocks(F &&f)
 {
@@ -43,6 +55,8 @@ __device__ __forceinline__ void for_blocks(F &&f)
 }
 
 // Maximum over each aligned group of G lanes; every lane gets its group's value.
+// DPP inside each 16-lane row, then the row maxima through scalar registers
+// (no LDS round trip).
 template <int G>
 __device__ __forceinline__ uint32_t group_max_u32(uint32_t v)
 {
@@ -52,54 +66,15 @@ __device__ __forceinline__ uint32_t group_max_u32(uint32_t v)
     v = max(v, dpp_u32<0x141>(v));  // row_half_mirror
     v = max(v, dpp_u32<0x140>(v));  // row_mirror
     if constexpr (G >= 32) {
-        // swap with the lane 16 apart inside each 32-lane half (bitmask swizzle, xor 0x10)
-        v = max(v, (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F));
-    }
-    if constexpr (G == 64) {
-        uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
-        uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
-        v = a > b ? a : b;
+        const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+        const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+        const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+        const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+        const uint32_t lo = r0 > r1 ? r0 : r1, hi = r2 > r3 ? r2 : r3;
+        if constexpr (G == 64) v = lo > hi ? lo : hi;
+        else v = (threadIdx.x & 32) ? hi : lo;
     }
     return v;
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v)
-{
-    int2 p = *reinterpret_cast<int2 *>(&v);
-    p.x = __builtin_amdgcn_update_dpp(0, p.x, CTRL, 0xF, 0xF, false);
-    p.y = __builtin_amdgcn_update_dpp(0, p.y, CTRL, 0xF, 0xF, false);
-    return *reinterpret_cast<double *>(&p);
-}
-
-// Value of lane `src` of each G-lane group, in every lane of the group.  src
-// is a constant after unrolling: 16-lane groups use DPP row_newbcast (each DPP
-// row is one group), wider groups a lane shuffle.
-template <int G>
-__device__ __forceinline__ double group_bcast(double v, int src)
-{
-    if constexpr (G == 16) {
-        switch (src) {
-        case 0: return dpp_f64<0x150>(v);
-        case 1: return dpp_f64<0x151>(v);
-        case 2: return dpp_f64<0x152>(v);
-        case 3: return dpp_f64<0x153>(v);
-        case 4: return dpp_f64<0x154>(v);
-        case 5: return dpp_f64<0x155>(v);
-        case 6: return dpp_f64<0x156>(v);
-        case 7: return dpp_f64<0x157>(v);
-        case 8: return dpp_f64<0x158>(v);
-        case 9: return dpp_f64<0x159>(v);
-        case 10: return dpp_f64<0x15A>(v);
-        case 11: return dpp_f64<0x15B>(v);
-        case 12: return dpp_f64<0x15C>(v);
-        case 13: return dpp_f64<0x15D>(v);
-        case 14: return dpp_f64<0x15E>(v);
-        default: return dpp_f64<0x15F>(v);
-        }
-    } else {
-        return __shfl(v, src, G);
-    }
 }
 
 template <int G>
@@ -130,6 +105,14 @@ __device__ __forceinline__ double upow(double b, int e)
     return p;
 }
 
+// a^2 + b^2 without contraction, so the matrix entries and the identity
+// operator's right-hand side round alike
+__device__ __forceinline__ double sq_sum(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a * a + b * b;
+}
+
 // |v|^m for the matrix block: odd m takes one square root
 __device__ __forceinline__ double phs_r2(int m, double r2)
 {
@@ -137,31 +120,47 @@ __device__ __forceinline__ double phs_r2(int m, double r2)
     return (m & 1) ? p * __builtin_sqrt(r2) : p;
 }
 
+// Powers of r = |v| shared by the closed-form operators at one point v.
+struct PhsAt {
+    double r2, rm2, rm4;  // r^2, r^(m-2), r^(m-4)
+};
+
+__device__ __forceinline__ PhsAt phs_at(int m, double vx, double vy)
+{
+    PhsAt q;
+    q.r2 = sq_sum(vx, vy);
+    q.rm2 = (m >= 2) ? phs_r2(m - 2, q.r2) : phs_r2(m, q.r2) / q.r2;
+    q.rm4 = q.rm2 / q.r2;
+    return q;
+}
+
 // Operator applied to the PHS at v: closed forms for the first- and
 // second-order operators, the general series for the others.
-__device__ inline double op_phs_fast(int op, int k, int m, double vx, double vy, double sx, double sy)
+__device__ __forceinline__ double op_phs_fast(int op, int k, int m, double vx, double vy, double sx, double sy,
+                                              const PhsAt &q)
 {
     if (op > OP_DXY && op != OP_LAP) return op_phs(op, k, m, vx, vy, sx, sy);
-    const double r2 = vx * vx + vy * vy;
     // E uses the matrix entries' own formula: for collocated rows b = A e_c holds
     // bit for bit, so the solve returns the identity row to rounding
-    if (op == OP_E) return phs_r2(m, r2);
-    const double rm2 = (m >= 2) ? phs_r2(m - 2, r2) : phs_r2(m, r2) / r2;  // r^(m-2)
-    const double rm4 = rm2 / r2;                                            // r^(m-4)
+    if (op == OP_E) return phs_r2(m, q.r2);
     const double dm = (double)m;
     switch (op) {
-    case OP_E:   return rm2 * r2;
-    case OP_DX:  return dm * vx * rm2;
-    case OP_DY:  return dm * vy * rm2;
-    case OP_DXX: return dm * rm2 + dm * (dm - 2.0) * vx * vx * rm4;
-    case OP_DYY: return dm * rm2 + dm * (dm - 2.0) * vy * vy * rm4;
-    case OP_DXY: return dm * (dm - 2.0) * vx * vy * rm4;
+    case OP_DX:  return dm * vx * q.rm2;
+    case OP_DY:  return dm * vy * q.rm2;
+    case OP_DXX: return dm * q.rm2 + dm * (dm - 2.0) * vx * vx * q.rm4;
+    case OP_DYY: return dm * q.rm2 + dm * (dm - 2.0) * vy * vy * q.rm4;
+    case OP_DXY: return dm * (dm - 2.0) * vx * vy * q.rm4;
     default: {  // OP_LAP with per-axis scales
-        const double fxx = dm * rm2 + dm * (dm - 2.0) * vx * vx * rm4;
-        const double fyy = dm * rm2 + dm * (dm - 2.0) * vy * vy * rm4;
+        const double fxx = dm * q.rm2 + dm * (dm - 2.0) * vx * vx * q.rm4;
+        const double fyy = dm * q.rm2 + dm * (dm - 2.0) * vy * vy * q.rm4;
         return sx * sx * fxx + sy * sy * fyy;
     }
     }
+}
+
+__device__ __forceinline__ double op_phs_fast(int op, int k, int m, double vx, double vy, double sx, double sy)
+{
+    return op_phs_fast(op, k, m, vx, vy, sx, sy, phs_at(m, vx, vy));
 }
 
 template <int NN, int NP, int R, int G>
@@ -173,9 +172,14 @@ struct LuShape {
     static constexpr int NODES = (NN + G - 1) / G; // stencil nodes per lane
     // U factor, packed by pivot step: row k holds columns k..WIDTH-1
     static constexpr int UTOT = NS * WIDTH - NS * (NS - 1) / 2;
+    // during assembly: P (node-major), then Phi as a packed upper triangle
+    static constexpr int PHI0 = NN * NP;
+    static constexpr int ASM = PHI0 + NN * (NN + 1) / 2;
+    // Phi(a, b), a <= b, at PHI0 + tri(a) + b
+    static constexpr __host__ __device__ int tri(int a) { return a * (2 * NN - a + 1) / 2 - a; }
     // +kStepBlock: a step writes up to kStepBlock-1 stale values past its row;
     // +2 keeps the groups' slots off one LDS bank
-    static constexpr int USTRIDE = cmax_<UTOT, NN * NP>::v + kStepBlock + 2;
+    static constexpr int USTRIDE = cmax_<UTOT, ASM>::v + kStepBlock + 2;
     static constexpr int NBLK = (NS + kStepBlock - 1) / kStepBlock;
     static constexpr __host__ __device__ int uoff(int k) { return k * WIDTH - k * (k - 1) / 2; }
 };
@@ -184,12 +188,14 @@ template <int NN, int NP, int R, int G>
 struct alignas(16) LuLDS {
     using T = LuShape<NN, NP, R, G>;
     double xi[T::S][NN][2];
-    double u[T::S][T::USTRIDE];  // monomial block during assembly, then the U factor
+    double u[T::S][T::USTRIDE];  // P and Phi during assembly, then the U factor
+    double rp[T::S];              // reciprocal of the current pivot
+    double x[T::S][R];            // the unknown solved last, per right-hand side
     int g[T::S][NN];
 };
 
-template <int NN, int NP, int R, int G, int WPB>
-__global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
+template <int NN, int NP, int R, int G, int WPB, int MINW>
+__global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArgs a)
 {
     using T = LuShape<NN, NP, R, G>;
     constexpr int NS = T::NS, ROWS = T::ROWS, WIDTH = T::WIDTH, S = T::S, NODES = T::NODES;
@@ -278,23 +284,40 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
             for (int q = 0; q < NP; ++q) L.u[grp][i * NP + q] = pxp[mono_ea(q)] * pyp[mono_eb(q)];
         }
     }
+    // Phi_ab = |xi_a - xi_b|^m once per pair: node i takes the pairs (i, i+d mod n)
+    // for d = 1..n/2, which covers each pair once (twice for d = n/2, n even)
+    {
+        double *phi = L.u[grp] + T::PHI0;
+        const double diag = phs_r2(m, 0.0);
+#pragma unroll
+        for (int r = 0; r < NODES; ++r) {
+            const int i = t + G * r;
+            if (i < NN) {
+                phi[T::tri(i) + i] = diag;
+#pragma unroll 3
+                for (int d = 1; d <= NN / 2; ++d) {
+                    int j = i + d;
+                    if (j >= NN) j -= NN;
+                    const double ddx = dxn[r] - L.xi[grp][j][0], ddy = dyn[r] - L.xi[grp][j][1];
+                    const int lo = i < j ? i : j, hi = i < j ? j : i;
+                    phi[T::tri(lo) + hi] = phs_r2(m, sq_sum(ddx, ddy));
+                }
+            }
+        }
+    }
     __builtin_amdgcn_wave_barrier();
 
-    // ---- rows of [A | B] (src/interpolationmatrix.jl:2-5, src/generate_operator.jl:157)
-    double row[ROWS][WIDTH];
+    // ---- right-hand sides first, while few registers are live: RBF rows at
+    // u = Yscaled - xi with exact zeros -> eps (src/generate_operator.jl:125-154),
+    // polynomial rows at Yscaled (:68-79)
+    double rhs[ROWS][R];
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
         const int i = t + G * rr;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) rhs[rr][cc] = 0.0;
         if (i < NN) {
             const double xix = L.xi[grp][i][0], xiy = L.xi[grp][i][1];
-#pragma unroll
-            for (int j = 0; j < NN; ++j) {
-                const double ddx = xix - L.xi[grp][j][0], ddy = xiy - L.xi[grp][j][1];
-                row[rr][j] = phs_r2(m, ddx * ddx + ddy * ddy);
-            }
-#pragma unroll
-            for (int q = 0; q < NP; ++q) row[rr][NN + q] = L.u[grp][i * NP + q];
-            // RBF rows of the right-hand sides at u = Yscaled - xi (exact zeros -> eps)
             double ux, uy;
             {
 #pragma clang fp contract(off)
@@ -308,25 +331,45 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
                     if (uy == 0.0) uy = kEps;
                 }
             }
+            const PhsAt q = phs_at(m, ux, uy);
 #pragma unroll
             for (int cc = 0; cc < R; ++cc)
-                row[rr][NS + cc] = cc < a.nops ? op_phs_fast(a.op_kind[cc], a.op_k[cc], m, ux, uy, sx, sy) : 0.0;
+                if (cc < a.nops) rhs[rr][cc] = op_phs_fast(a.op_kind[cc], a.op_k[cc], m, ux, uy, sx, sy, q);
+        } else if (i < NS) {
+            const int q = i - NN;
+            int tt = 0;
+            while ((tt + 1) * (tt + 2) / 2 <= q) ++tt;
+            const int ea = q - tt * (tt + 1) / 2, eb = tt - ea;
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc)
+                if (cc < a.nops) rhs[rr][cc] = op_mono(a.op_kind[cc], a.op_k[cc], ea, eb, yx, yy, sx, sy);
+        }
+    }
+
+    // ---- rows of [A | B] (src/interpolationmatrix.jl:2-5, src/generate_operator.jl:157)
+    double row[ROWS][WIDTH];
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr) {
+        const int i = t + G * rr;
+        if (i < NN) {
+            const double *phi = L.u[grp] + T::PHI0;
+            const int rbase = T::tri(i);  // Phi(i, j) for j >= i
+#pragma unroll
+            for (int j = 0; j < NN; ++j) row[rr][j] = phi[j >= i ? rbase + j : T::tri(j) + i];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) row[rr][NN + q] = L.u[grp][i * NP + q];
         } else if (i < NS) {
             const int q = i - NN;
 #pragma unroll
             for (int j = 0; j < NN; ++j) row[rr][j] = L.u[grp][j * NP + q];
 #pragma unroll
             for (int j = NN; j < NS; ++j) row[rr][j] = 0.0;
-            int tt = 0;
-            while ((tt + 1) * (tt + 2) / 2 <= q) ++tt;
-            const int ea = q - tt * (tt + 1) / 2, eb = tt - ea;
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc)
-                row[rr][NS + cc] = cc < a.nops ? op_mono(a.op_kind[cc], a.op_k[cc], ea, eb, yx, yy, sx, sy) : 0.0;
         } else {
 #pragma unroll
-            for (int j = 0; j < WIDTH; ++j) row[rr][j] = 0.0;
+            for (int j = 0; j < NS; ++j) row[rr][j] = 0.0;
         }
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) row[rr][NS + cc] = rhs[rr][cc];
     }
     __builtin_amdgcn_wave_barrier();
 
@@ -342,6 +385,33 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
         if (t + G * rr >= NS) done |= 1u << rr;
     bool singular = false;
     double *U = L.u[grp];
+    // This lane's pivot candidate: its largest |row[.][0]| among rows not yet
+    // eliminated, as a key whose low 7 bits name the row, and the reciprocal of
+    // that entry, so the winning lane publishes 1/pivot with the pivot row.
+    uint32_t key;
+    double crp;
+    auto candidate = [&]() {
+        uint32_t kk = 0;
+        double cv = 1.0;
+#pragma unroll
+        for (int rr = 0; rr < ROWS; ++rr) {
+            if (!((done >> rr) & 1u)) {
+                const float f = __builtin_fabsf((float)row[rr][0]);
+                const uint32_t kr = (__float_as_uint(f) & 0xFFFFFF00u) | 0x80u | (uint32_t)(127 - (t + G * rr));
+                if (kr > kk) {
+                    kk = kr;
+                    cv = row[rr][0];
+                }
+            }
+        }
+        double r = __builtin_amdgcn_rcp(cv);
+        r = __builtin_fma(r, __builtin_fma(-cv, r, 1.0), r);
+        r = __builtin_fma(r, __builtin_fma(-cv, r, 1.0), r);
+        key = kk;
+        crp = r;
+    };
+    candidate();
+    uint32_t kmax = group_max_u32<G>(key);
     int k = 0;
     for_blocks<0, T::NBLK>([&](auto bc) {
         constexpr int b = decltype(bc)::value;
@@ -349,16 +419,6 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
         const int kend = NS < kStepBlock * (b + 1) ? NS : kStepBlock * (b + 1);
 #pragma nounroll
         for (; k < kend; ++k) {
-            uint32_t key = 0;
-#pragma unroll
-            for (int rr = 0; rr < ROWS; ++rr) {
-                if (!((done >> rr) & 1u)) {
-                    const float f = __builtin_fabsf((float)row[rr][0]);
-                    const uint32_t kr = (__float_as_uint(f) & 0xFFFFFF00u) | 0x80u | (uint32_t)(127 - (t + G * rr));
-                    key = key > kr ? key : kr;
-                }
-            }
-            const uint32_t kmax = group_max_u32<G>(key);
             const int v = 127 - (int)(kmax & 127u);
             const bool iam = t == v % G;
             const int psel = v / G;
@@ -371,21 +431,27 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
                     done |= 1u << rr;
                 }
             }
+            if (iam) L.rp[grp] = crp;
             // the other lanes read what the pivot lane just wrote
             __builtin_amdgcn_wave_barrier();
-            const double p = piv[0];
-            singular |= (p == 0.0);
-            double rp = __builtin_amdgcn_rcp(p);
-            rp = __builtin_fma(rp, __builtin_fma(-p, rp, 1.0), rp);
-            rp = __builtin_fma(rp, __builtin_fma(-p, rp, 1.0), rp);
-            double l[ROWS];
+            const double rp = L.rp[grp];
+            singular |= (piv[0] == 0.0);
+            double nl[ROWS];
 #pragma unroll
-            for (int rr = 0; rr < ROWS; ++rr) l[rr] = ((done >> rr) & 1u) ? 0.0 : row[rr][0] * rp;
+            for (int rr = 0; rr < ROWS; ++rr) nl[rr] = ((done >> rr) & 1u) ? -0.0 : -(row[rr][0] * rp);
+            // column k+1 first: the next step's pivot search overlaps the rest
+            {
+                const double p1 = piv[1];
 #pragma unroll
-            for (int j = 0; j + 1 < WB; ++j) {
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
+            }
+            candidate();
+            kmax = group_max_u32<G>(key);
+#pragma unroll
+            for (int j = 1; j + 1 < WB; ++j) {
                 const double pj = piv[j + 1];
 #pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = __builtin_fma(-l[rr], pj, row[rr][j + 1]);
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], pj, row[rr][j + 1]);
             }
             // the next step's pivot lane overwrites the stale tail read here
             __builtin_amdgcn_wave_barrier();
@@ -394,10 +460,12 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
     __builtin_amdgcn_wave_barrier();
 
     // ---- back substitution from the U factor in LDS.  Lane t owns the
-    // unknowns i = t + G s: it holds y_i, forms x_i = y_i / U_ii when its turn
-    // comes, and a group broadcast of x_k lets every lane update the y_i it
-    // owns (i < k).  The lane that owns x_i is the one that writes node i.
-    double yv[ROWS][R], xv[ROWS][R], rdiag[ROWS];
+    // unknowns i = t + G s and holds y_i.  At step k the owner of x_k writes
+    // x_k = y_k / U_kk to LDS and every lane updates the y_i it owns (i < k).
+    // A y_i is final after its own step, so x_i = y_i / U_ii once the sweep
+    // ends, bit for bit the value written at its step.  The lane that owns
+    // x_i is the one that writes node i.
+    double yv[ROWS][R], rdiag[ROWS];
     int ubase[ROWS];
 #pragma unroll
     for (int s2 = 0; s2 < ROWS; ++s2) {
@@ -406,33 +474,39 @@ __global__ __launch_bounds__(64 * WPB) void solve_lu_kernel(const SolveArgs a)
         ubase[s2] = T::uoff(ic) - ic;  // U(i, j) = U[ubase + j]
         rdiag[s2] = 1.0 / U[ubase[s2] + ic];
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) {
-            yv[s2][cc] = U[ubase[s2] + NS + cc];
-            xv[s2][cc] = 0.0;
-        }
+        for (int cc = 0; cc < R; ++cc) yv[s2][cc] = U[ubase[s2] + NS + cc];
     }
+    double *xsol = L.x[grp];
+#pragma unroll
+    for (int s = ROWS - 1; s >= 0; --s) {
+        const int tend = s == ROWS - 1 ? NS - 1 - G * s : G - 1;
 #pragma nounroll
-    for (int kk = NS - 1; kk >= 0; --kk) {
-        const int sk = kk / G, tk = kk % G;
-        double uik[ROWS];
+        for (int tk = tend; tk >= 0; --tk) {
+            const int kk = G * s + tk;
+            if (t == tk) {
 #pragma unroll
-        for (int s2 = 0; s2 < ROWS; ++s2) {
-            const int i = t + G * s2;
-            uik[s2] = (i < kk) ? U[ubase[s2] + kk] : 0.0;
-        }
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) {
-            double own = 0.0;
-#pragma unroll
-            for (int s2 = 0; s2 < ROWS; ++s2) own = (s2 == sk) ? yv[s2][cc] * rdiag[s2] : own;
-            const double xk = __shfl(own, tk, G);
-#pragma unroll
-            for (int s2 = 0; s2 < ROWS; ++s2) {
-                xv[s2][cc] = (s2 == sk && t == tk) ? xk : xv[s2][cc];
-                yv[s2][cc] = __builtin_fma(-uik[s2], xk, yv[s2][cc]);
+                for (int cc = 0; cc < R; ++cc) xsol[cc] = yv[s][cc] * rdiag[s];
             }
+            __builtin_amdgcn_wave_barrier();
+            double xk[R];
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) xk[cc] = xsol[cc];
+            // rows of earlier slots all precede k; in slot s only t < tk does
+#pragma unroll
+            for (int s2 = 0; s2 <= s; ++s2) {
+                const double uik = (s2 < s || t < tk) ? U[ubase[s2] + kk] : 0.0;
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) yv[s2][cc] = __builtin_fma(-uik, xk[cc], yv[s2][cc]);
+            }
+            // the next step's owner overwrites the values read here
+            __builtin_amdgcn_wave_barrier();
         }
     }
+    double xv[ROWS][R];
+#pragma unroll
+    for (int s2 = 0; s2 < ROWS; ++s2)
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) xv[s2][cc] = yv[s2][cc] * rdiag[s2];
 
     // ---- CSR output: sorted columns by rank inside the stencil
     uint32_t bad = 0;
