@@ -6,16 +6,22 @@ nearest-node map, and the local solves of r^7 / degree-4 systems for the
 operators d/dx, d/dy and the Laplacian, written as FP64 CSR rows.
 
 The JSON line reports:
-  value                stencils/s of the whole step, FP64 solver (the bar:
-                       weights within 1e-10 of the oracle)
+  value                stencils/s of the whole step, FP64 LU solver (weights
+                       within 1e-10 of the high-precision oracle)
   roofline             the local-solve kernel: algorithmic flops per launch
                        ((2/3) Nsys^3 + 2 Nsys^2 r per stencil, Nsys = 46, r = 3)
-                       over its HIP-event duration, against the FP64 peak
+                       over its HIP-event duration, against the FP64 peak;
+                       traffic = HBM bytes per launch from the rocprofv3 PMC
+                       summary committed with the sources
   cpu_baseline         the oracle (a C restatement of the reference's
-                       per-stencil algorithm, inv(A) * RHS) on the host cores,
-                       over a bounded sample of the same rows
-  spmv / spmm          y = L x for the generated Laplacian (1 and 8 fields),
-                       algorithmic bytes over HIP-event time
+                       per-stencil algorithm, inv(A) * RHS) on the host cores
+                       over all rows of the workload; the same pass gives the
+                       GPU weights' max and p99 row error per operator
+  end_to_end           the host-level call (host arrays in, CSC out): H2D,
+                       kNN, solves, CSR->CSC, D2H
+  spmv                 y = L x for the generated Laplacian (1 and 8 fields),
+                       algorithmic bytes over HIP-event time, beside a
+                       device-to-device copy of 1 GiB
   mixed                the FP32 Gauss-Jordan + FP64 refinement solver, same
                        workload, and its ratio to the FP64 solver
 """
@@ -35,6 +41,7 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "radialbasisfinitedifferences.jl_amd")]
 METRIC = "stencils/s (1M nodes, deg-4 PHS); SpMV GB/s; mixed vs FP64 at ≤1e-10"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+OPS = ["Dx", "Dy", "Lap"]
 
 
 def log(*a):
@@ -42,7 +49,7 @@ def log(*a):
 
 
 def read_traffic(name):
-    """HBM bytes per launch recorded by the rocprofv3 --pmc summaries."""
+    """HBM bytes per launch recorded by a rocprofv3 --pmc summary."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -55,13 +62,20 @@ def read_traffic(name):
     return None
 
 
+def row_errors(got, ref):
+    """max and p99 of the per-row relative error max_j |got - ref| / max_j |ref|."""
+    e = np.abs(got - ref).max(axis=1) / np.maximum(np.abs(ref).max(axis=1), 1e-300)
+    return float(e.max()), float(np.percentile(e, 99))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the host-level end-to-end leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1"}))
@@ -77,42 +91,43 @@ def main():
     X, prm = config_nodes("gen1m")
     p, polydeg, n = prm["p"], prm["polydeg"], prm["n"]
     N = len(X)
-    ops = ["Dx", "Dy", "Lap"]
     Xd = ctx.to_device(X)
-    col = ctx.empty((N, n), np.int32)
-    vals = [ctx.empty((N, n), np.float64) for _ in ops]
-    st_d = ctx.empty((N, n), np.int32)
     P = (polydeg + 1) * (polydeg + 2) // 2
     nsys = n + P
-    flops_per_stencil = (2.0 / 3.0) * nsys**3 + 2.0 * nsys**2 * len(ops)
+    flops_per_stencil = (2.0 / 3.0) * nsys**3 + 2.0 * nsys**2 * len(OPS)
 
-    def run(precision, steps, warmup, tag):
+    def run(precision, steps, warmup, tag, col, vals, st_d):
+        stats = None
         for _ in range(warmup):
-            eng.generate(Xd, Xd, p, n, polydeg, ops, precision=precision, colidx=col, vals=vals,
+            eng.generate(Xd, Xd, p, n, polydeg, OPS, precision=precision, colidx=col, vals=vals,
                          stencils_out=st_d)
         ctx.synchronize()
-        ms_solve, ms_knn, ms_fb, fb = [], [], [], 0
+        ms_solve, ms_knn, ms_fb = [], [], []
         t0 = time.perf_counter()
         ctx.timer_start()
-        for s in range(steps):
-            _, _, stats = eng.generate(Xd, Xd, p, n, polydeg, ops, precision=precision, colidx=col,
+        for _ in range(steps):
+            _, _, stats = eng.generate(Xd, Xd, p, n, polydeg, OPS, precision=precision, colidx=col,
                                        vals=vals, stencils_out=st_d)
             ms_solve.append(stats["ms_solve"])
             ms_knn.append(stats["ms_knn"])
             ms_fb.append(stats["ms_fallback"])
-            fb = stats["refined_fallback"]
         ms = ctx.timer_stop()
         ctx.synchronize()
         wall = (time.perf_counter() - t0) * 1e3
         log(f"[{tag}] {steps} steps: {ms / steps:.3f} ms/step (wall {wall / steps:.3f}), "
-            f"knn {np.mean(ms_knn):.3f} ms, solve {np.mean(ms_solve):.3f} ms, fallback rows {fb}")
-        return ms / steps, float(np.mean(ms_solve)), float(np.mean(ms_knn)), float(np.mean(ms_fb)), fb
+            f"knn {np.mean(ms_knn):.3f} ms, solve {np.mean(ms_solve):.3f} ms, "
+            f"fallback rows {stats['refined_fallback']}, lanes/stencil {stats['lanes_per_stencil']}")
+        return {"ms_step": ms / steps, "ms_solve": float(np.mean(ms_solve)), "ms_knn": float(np.mean(ms_knn)),
+                "ms_fallback": float(np.mean(ms_fb)), "fallback_rows": int(stats["refined_fallback"]),
+                "lanes": int(stats["lanes_per_stencil"])}
 
+    col = ctx.empty((N, n), np.int32)
+    vals = [ctx.empty((N, n), np.float64) for _ in OPS]
+    st_d = ctx.empty((N, n), np.int32)
     log("timing FP64 solver")
-    ms_step, ms_solve, ms_knn, _, _ = run("fp64", args.steps, args.warmup, "fp64")
-    value = N / (ms_step * 1e-3)
-    achieved = N * flops_per_stencil / (ms_solve * 1e-3) / 1e12
-
+    r64 = run("fp64", args.steps, args.warmup, "fp64", col, vals, st_d)
+    value = N / (r64["ms_step"] * 1e-3)
+    achieved = N * flops_per_stencil / (r64["ms_solve"] * 1e-3) / 1e12
     colh = col.get()
     wh = [v.get() for v in vals]
 
@@ -120,18 +135,28 @@ def main():
     if not args.no_mixed:
         log("timing mixed solver")
         mcol = ctx.empty((N, n), np.int32)
-        mvals = [ctx.empty((N, n), np.float64) for _ in ops]
-        col_fp64, vals_fp64 = col, vals
-        col, vals = mcol, mvals
-        msm, mss, _, msfb, fb = run("mixed", args.steps, args.warmup, "mixed")
-        col, vals = col_fp64, vals_fp64
+        mvals = [ctx.empty((N, n), np.float64) for _ in OPS]
+        rm = run("mixed", args.steps, args.warmup, "mixed", mcol, mvals, st_d)
         mh = [v.get() for v in mvals]
-        dev = max(float((np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)).max()) for a, b in zip(mh, wh))
-        mixed = {"stencils_per_s": N / (msm * 1e-3), "ms_per_step": msm, "ms_solve": mss, "ms_fallback": msfb,
-                 "fallback_rows": int(fb), "max_row_dev_vs_fp64": dev,
-                 "mixed_vs_fp64": (N / (msm * 1e-3)) / value}
+        dev = max(row_errors(a, b)[0] for a, b in zip(mh, wh))
+        mixed = {"stencils_per_s": N / (rm["ms_step"] * 1e-3), "ms_per_step": rm["ms_step"],
+                 "ms_solve": rm["ms_solve"], "ms_fallback": rm["ms_fallback"], "fallback_rows": rm["fallback_rows"],
+                 "max_row_dev_vs_fp64": dev, "mixed_vs_fp64": (N / (rm["ms_step"] * 1e-3)) / value}
+        del mcol, mvals
 
-    # operator application: Laplacian, 1 and 8 fields
+    e2e = None
+    if not args.no_e2e:
+        log("timing the host-level call (H2D + kNN + solves + CSR->CSC + D2H)")
+        rbffd_amd.operators(X[:4096], X[:4096], p, n, polydeg, OPS, ctx=ctx)  # warm the small paths
+        t0 = time.perf_counter()
+        mats = rbffd_amd.operators(X, X, p, n, polydeg, OPS, ctx=ctx)
+        dt = time.perf_counter() - t0
+        e2e = {"stencils_per_s": N / dt, "seconds": dt,
+               "note": "host arrays in, SparseMatrixCSC-layout arrays out, PCIe included"}
+        log(f"  end to end {dt * 1e3:.1f} ms, {N / dt:.3e} stencils/s")
+        del mats
+
+    # operator application: the Laplacian on 1 and 8 fields, and a plain copy
     log("timing SpMV / SpMM")
     nnz = N * n
     spmv = {}
@@ -151,53 +176,86 @@ def main():
         spmv[f"F{F}"] = {"ms": t, "GB_s": bytes_alg / (t * 1e-3) / 1e9, "bytes": bytes_alg,
                          "frac_hbm_peak": bytes_alg / (t * 1e-3) / 1e9 / PEAK_HBM_GBS}
         log(f"  spmv F={F}: {t * 1e3:.1f} us, {spmv[f'F{F}']['GB_s']:.0f} GB/s")
+    a = ctx.empty((1 << 27,), np.float64)
+    b = ctx.empty((1 << 27,), np.float64)
+    b.copy_from_device(a)
+    reps = 10
+    ctx.timer_start()
+    for _ in range(reps):
+        b.copy_from_device(a)
+    t = ctx.timer_stop() / reps
+    copy_gbs = 2 * a.nbytes / (t * 1e-3) / 1e9
+    spmv["copy_1GiB_GB_s"] = copy_gbs
+    log(f"  device copy of 1 GiB: {t:.3f} ms, {copy_gbs:.0f} GB/s (read + write)")
+    del a, b
 
     cpu = None
+    accuracy = None
     if not args.no_cpu:
-        log("CPU baseline (oracle restatement)")
+        log("CPU baseline (oracle restatement, all rows)")
         from oracle import oracle as O
 
         threads = min(os.cpu_count() or 1, 16)
-        rows = np.arange(0, N, 8, dtype=np.int64)  # every 8th row: 131,072 stencils
         t0 = time.perf_counter()
-        st, _ = O.knn(X, X[rows], n, nthreads=threads)
-        W, _, status = O.weights(X, X, _scatter(st, rows, N, n), None, p, polydeg,
-                                 [(O.OP_DX, 0), (O.OP_DY, 0), (O.OP_LAP, 0)], rows=rows, nthreads=threads)
+        st, _ = O.knn(X, X, n, nthreads=threads)
+        W, _, status = O.weights(X, X, st, None, p, polydeg, [(O.OP_DX, 0), (O.OP_DY, 0), (O.OP_LAP, 0)],
+                                 nthreads=threads)
         dt = time.perf_counter() - t0
-        # the GPU rows of the same sample, against the oracle (relative row error)
-        gst = np.sort(st, axis=1)
-        same_pattern = bool(np.array_equal(colh[rows], gst))
-        errs = []
-        for o in range(3):
-            ref = np.take_along_axis(W[:, o, :], np.argsort(st, axis=1, kind="stable"), 1)
-            errs.append(float((np.abs(wh[o][rows] - ref).max(axis=1) / np.abs(ref).max(axis=1)).max()))
-        cpu = {"value": len(rows) / dt, "unit": "stencils/s", "cores": threads, "kind": "port",
-               "sample": f"{len(rows)} of {N} rows (every 8th), kNN + weights, {dt:.1f} s",
-               "gpu_pattern_equal": same_pattern, "gpu_max_row_err_vs_oracle": max(errs)}
-        log(f"  CPU {cpu['value']:.0f} stencils/s on {threads} threads; GPU err vs oracle {max(errs):.2e}")
+        same_pattern = bool(np.array_equal(colh, np.sort(st, axis=1)))
+        order = np.argsort(st, axis=1, kind="stable")
+        accuracy = {"vs_fp64_oracle_inv": {}}
+        for o, name in enumerate(OPS):
+            ref = np.take_along_axis(W[:, o, :], order, 1)
+            mx, p99 = row_errors(wh[o], ref)
+            accuracy["vs_fp64_oracle_inv"][name] = {"max": mx, "p99": p99}
+        del W
+        # the high-precision oracle on a sample of rows
+        rows = np.arange(0, N, 256, dtype=np.int64)
+        Wq, _, _ = O.weights(X, X, st, None, p, polydeg, [(O.OP_DX, 0), (O.OP_DY, 0), (O.OP_LAP, 0)],
+                             quad=True, rows=rows, nthreads=threads)
+        accuracy["vs_quad_oracle"] = {"rows": len(rows)}
+        for o, name in enumerate(OPS):
+            ref = np.take_along_axis(Wq[:, o, :], order[rows], 1)
+            mx, p99 = row_errors(wh[o][rows], ref)
+            accuracy["vs_quad_oracle"][name] = {"max": mx, "p99": p99}
+        cpu = {"value": N / dt, "unit": "stencils/s", "cores": threads, "kind": "port",
+               "sample": f"all {N} rows of the workload, kNN + weights (inv(A) * RHS), {dt:.1f} s wall",
+               "gpu_pattern_equal": same_pattern}
+        log(f"  CPU {cpu['value']:.0f} stencils/s on {threads} threads; pattern equal {same_pattern}")
+        log(f"  accuracy {json.dumps(accuracy)}")
+        # CPU SpMV of the same Laplacian, one thread and all threads
+        rowptr = np.arange(0, nnz + 1, n, dtype=np.int64)
+        xv = np.random.default_rng(1).standard_normal(N)
+        cpu_spmv = {}
+        for th in (1, threads):
+            O.spmv_csr(rowptr, colh, wh[2], xv, nthreads=th)
+            t0 = time.perf_counter()
+            reps = 5
+            for _ in range(reps):
+                O.spmv_csr(rowptr, colh, wh[2], xv, nthreads=th)
+            ts = (time.perf_counter() - t0) / reps
+            cpu_spmv[f"threads_{th}"] = {"ms": ts * 1e3, "GB_s": (nnz * 12 + 8 * (N + 1) + 16 * N) / ts / 1e9}
+        cpu["spmv"] = cpu_spmv
+        log(f"  CPU SpMV {json.dumps(cpu_spmv)}")
 
+    G = r64["lanes"]
     out = {
         "metric": METRIC, "value": value, "unit": "stencils/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": r64["ms_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "gen1m: 1,048,576 jittered nodes (1024x1024, +-0.25h, seed 2), PHS r^7, "
                                "degree-4 monomials, n=31, operators Dx, Dy, Lap as FP64 CSR",
-                   "nodes": N, "stencil": n, "polydeg": polydeg, "phs_power": p, "ops": ops},
-        "roofline": {"bound": "mfma", "pipe": "valu-fp64", "kernel": "solve_kernel<double,31,15,3>",
+                   "nodes": N, "stencil": n, "polydeg": polydeg, "phs_power": p, "ops": OPS},
+        "roofline": {"bound": "mfma", "pipe": "valu-fp64", "kernel": f"solve_lu_kernel<31,15,3,{G}>",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": read_traffic("solve_gen1m_pmc.txt")},
-        "kernel_only_stencils_per_s": N / (ms_solve * 1e-3),
-        "ms_knn": ms_knn, "ms_solve": ms_solve,
+                     "traffic": read_traffic("solve_lu_gen1m_pmc.txt")},
+        "kernel_only_stencils_per_s": N / (r64["ms_solve"] * 1e-3),
+        "ms_knn": r64["ms_knn"], "ms_solve": r64["ms_solve"], "lanes_per_stencil": G,
+        "end_to_end": e2e, "accuracy": accuracy,
         "spmv": spmv, "mixed": mixed, "cpu_baseline": cpu, "device": info,
     }
     print(json.dumps(out))
-
-
-def _scatter(st_rows, rows, N, n):
-    full = np.zeros((N, n), np.int32)
-    full[rows] = st_rows
-    return full
 
 
 if __name__ == "__main__":

@@ -10,8 +10,10 @@
 // harnesses (test/poisson_test.jl:121, examples/adv_diff_test.jl:151-152).
 // A group of G lanes owns one row: the row's values and columns are read as
 // contiguous runs (coalesced), x is gathered per column, and the G partial
-// sums are reduced across lanes.  Fields are stored node-major (x[col*F + f])
-// so one gather brings all F fields of a node.
+// sums are reduced across lanes by DPP.  Fields are stored node-major
+// (x[col*F + f]) so one gather brings all F fields of a node.
+#include <cstdlib>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -123,6 +125,31 @@ void launch_f64_to_f32(const double *src, long n, float *dst, hipStream_t s)
 }
 
 // ------------------------------------------------------------------ SpMV/SpMM
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    int2 p = *reinterpret_cast<int2 *>(&v);
+    p.x = __builtin_amdgcn_update_dpp(0, p.x, CTRL, 0xF, 0xF, false);
+    p.y = __builtin_amdgcn_update_dpp(0, p.y, CTRL, 0xF, 0xF, false);
+    return *reinterpret_cast<double *>(&p);
+}
+
+// Sum over each aligned group of G lanes, in every lane of the group: DPP
+// inside a 16-lane row (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror), a shuffle across rows.  Every lane adds the
+// same two partial sums, so all lanes hold the same bits.
+template <int G>
+__device__ __forceinline__ double group_sum_f64(double v)
+{
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32, "group size");
+    if constexpr (G >= 2) v += dpp_f64<0xB1>(v);
+    if constexpr (G >= 4) v += dpp_f64<0x4E>(v);
+    if constexpr (G >= 8) v += dpp_f64<0x141>(v);
+    if constexpr (G >= 16) v += dpp_f64<0x140>(v);
+    if constexpr (G >= 32) v += __shfl_xor(v, 16, 32);
+    return v;
+}
+
 template <typename TV, int G, int F, bool UNIFORM>
 __global__ __launch_bounds__(256) void spmv_kernel(long M, const int64_t *__restrict__ rowptr,
                                                    const int *__restrict__ col, const TV *__restrict__ val,
@@ -149,10 +176,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(long M, const int64_t *__rest
         for (int f = 0; f < F; ++f) acc[f] = __builtin_fma(v, x[cidx + f], acc[f]);
     }
 #pragma unroll
-    for (int o = G / 2; o >= 1; o >>= 1) {
-#pragma unroll
-        for (int f = 0; f < F; ++f) acc[f] += __shfl_xor(acc[f], o, G);
-    }
+    for (int f = 0; f < F; ++f) acc[f] = group_sum_f64<G>(acc[f]);
     if (live) {
         // the G lanes of a row share the F stores
 #pragma unroll
@@ -178,15 +202,38 @@ static void spmv_launch_t(long M, const int64_t *rowptr, const int *col, const T
         spmv_kernel<TV, G, F, false><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, x, y, alpha, beta);
 }
 
+// Lanes per row: 16 (one field or several), or RBFFD_SPMV_LANES /
+// RBFFD_SPMM_LANES = 4 | 8 | 16 | 32.
+static int lanes_env(const char *name, int dflt)
+{
+    const char *e = std::getenv(name);
+    const int v = e ? std::atoi(e) : dflt;
+    return (v == 4 || v == 8 || v == 16 || v == 32) ? v : dflt;
+}
+
+template <typename TV, int F>
+static void spmv_lanes_t(int G, long M, const int64_t *rowptr, const int *col, const TV *val, int nu,
+                         const double *x, double *y, double alpha, double beta, hipStream_t s)
+{
+    switch (G) {
+    case 4: spmv_launch_t<TV, 4, F>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    case 8: spmv_launch_t<TV, 8, F>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    case 32: spmv_launch_t<TV, 32, F>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    default: spmv_launch_t<TV, 16, F>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    }
+}
+
 template <typename TV>
 static void spmv_dispatch(long M, const int64_t *rowptr, const int *col, const TV *val, int F, int nu,
                           const double *x, double *y, double alpha, double beta, hipStream_t s)
 {
+    static const int g1 = lanes_env("RBFFD_SPMV_LANES", 16);
+    static const int gf = lanes_env("RBFFD_SPMM_LANES", 16);
     switch (F) {
-    case 1: spmv_launch_t<TV, 16, 1>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    case 2: spmv_launch_t<TV, 16, 2>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    case 4: spmv_launch_t<TV, 16, 4>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    case 8: spmv_launch_t<TV, 16, 8>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    case 1: spmv_lanes_t<TV, 1>(g1, M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    case 2: spmv_lanes_t<TV, 2>(gf, M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    case 4: spmv_lanes_t<TV, 4>(gf, M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    case 8: spmv_lanes_t<TV, 8>(gf, M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
     default: throw Error{2, "spmv: fields must be 1, 2, 4 or 8"};
     }
 }

@@ -94,6 +94,11 @@ class DeviceArray:
             raise ValueError("size mismatch")
         check(_lib.load().rbffd_memcpy_h2d(self.ctx.handle, self._p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
 
+    def copy_from_device(self, other: "DeviceArray"):
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        check(_lib.load().rbffd_memcpy_d2d(self.ctx.handle, self._p, other.ptr, self.nbytes))
+
     def get(self):
         out = np.empty(self.shape, self.dtype)
         check(_lib.load().rbffd_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p), self._p,
