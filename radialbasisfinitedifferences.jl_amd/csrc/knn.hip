@@ -44,22 +44,35 @@ __global__ void bbox_partial_kernel(const double *X, long N, double *partials)
     if (threadIdx.x < 5) partials[blockIdx.x * 5 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
-__global__ void bbox_final_kernel(const double *partials, int nblocks, double *box)
+__global__ __launch_bounds__(256) void bbox_final_kernel(const double *partials, int nblocks, double *box)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double sm[5][256];
     double a = INFINITY, b = INFINITY, c = -INFINITY, d = -INFINITY, bad = 0.0;
-    for (int i = 0; i < nblocks; ++i) {
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
         a = fmin(a, partials[5 * i]); b = fmin(b, partials[5 * i + 1]);
         c = fmax(c, partials[5 * i + 2]); d = fmax(d, partials[5 * i + 3]);
         bad += partials[5 * i + 4];
     }
-    box[0] = a; box[1] = b; box[2] = c; box[3] = d; box[4] = bad;
+    sm[0][threadIdx.x] = a; sm[1][threadIdx.x] = b; sm[2][threadIdx.x] = c; sm[3][threadIdx.x] = d;
+    sm[4][threadIdx.x] = bad;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sm[0][threadIdx.x] = fmin(sm[0][threadIdx.x], sm[0][threadIdx.x + o]);
+            sm[1][threadIdx.x] = fmin(sm[1][threadIdx.x], sm[1][threadIdx.x + o]);
+            sm[2][threadIdx.x] = fmax(sm[2][threadIdx.x], sm[2][threadIdx.x + o]);
+            sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + o]);
+            sm[4][threadIdx.x] += sm[4][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) box[threadIdx.x] = sm[threadIdx.x][0];
 }
 
 void launch_bbox(const double *X, long N, double *box, double *partials, int nblocks, hipStream_t s)
 {
     bbox_partial_kernel<<<nblocks, 256, 0, s>>>(X, N, partials);
-    bbox_final_kernel<<<1, 64, 0, s>>>(partials, nblocks, box);
+    bbox_final_kernel<<<1, 256, 0, s>>>(partials, nblocks, box);
 }
 
 __device__ __forceinline__ int cell_coord(double v, double v0, double cs, int g)

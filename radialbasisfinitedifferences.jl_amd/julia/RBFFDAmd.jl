@@ -1,0 +1,153 @@
+# RBFFDAmd.jl -- Julia binding of librbffd (include/rbffd.h) with the API of
+# RadialBasisFiniteDifferences.jl's operator path:
+#
+#   generate_operator(X, Y, p, n, polydeg)                       src/generate_operator.jl:29
+#   generate_operator(X, Y, p, n, polydeg, X_idx_in, X_idx_bc, X_idx_bc_g,
+#                     Y_idx_in, Y_idx_bc, Y_idx_bc_g)            src/generate_operator.jl:192-193
+#   generate_operator(X, p, n, polydeg)                          src/generate_operator.jl:354
+#   hyperviscosity_operator(k, X, Y, p, n, polydeg[, six index sets])
+#                                                                src/hyperviscosity_operator.jl:26,177-179
+#   hyperviscosity_operator(k, X, p, n, polydeg)                 src/hyperviscosity_operator.jl:314
+#
+# Each returns SparseMatrixCSC{Float64,Int64} in the reference's layout:
+# 1-based, rows sorted inside each column, explicit zeros kept, dims
+# (number of Y nodes, largest referenced column).  Errors map to the
+# exceptions the reference raises (SingularException from inv, ArgumentError,
+# UndefRefError for a node in no index set).
+#
+# Set ENV["RBFFD_LIB"] to the path of librbffd.so.  One context per Julia
+# thread; calls are synchronous.
+module RBFFDAmd
+
+using LinearAlgebra
+using SparseArrays
+using StaticArrays
+
+export generate_operator, hyperviscosity_operator
+
+const LIB = get(ENV, "RBFFD_LIB", joinpath(@__DIR__, "..", "rbffd_amd", "librbffd.so"))
+
+# operator kinds and precisions (include/rbffd.h)
+const OP_E, OP_DX, OP_DY, OP_DXX, OP_DYY, OP_DXY, OP_XK, OP_YK = Int32.(0:7)
+const PREC_FP64 = Int32(0)
+
+# struct rbffd_boundary (same field order and C alignment)
+struct Boundary
+    index_base::Int32
+    n_in::Int64
+    in_idx::Ptr{Int64}
+    n_markers::Int32
+    bc_ptr::Ptr{Int64}
+    bc_idx::Ptr{Int64}
+    ghost_ptr::Ptr{Int64}
+    ghost_idx::Ptr{Int64}
+end
+
+# struct rbffd_stats
+struct Stats
+    rows::Int64
+    singular::Int64
+    nonfinite::Int64
+    refined_fallback::Int64
+    ms_knn::Float64
+    ms_solve::Float64
+    ms_fallback::Float64
+    lanes_per_stencil::Int64
+end
+
+const CONTEXTS = Dict{Int,Ptr{Cvoid}}()
+const CONTEXT_LOCK = ReentrantLock()
+
+function context()
+    tid = Threads.threadid()
+    lock(CONTEXT_LOCK) do
+        get!(CONTEXTS, tid) do
+            h = Ref{Ptr{Cvoid}}(C_NULL)
+            check(ccall((:rbffd_create, LIB), Cint, (Cint, Ref{Ptr{Cvoid}}), 0, h))
+            h[]
+        end
+    end
+end
+
+function check(code::Integer)
+    code == 0 && return nothing
+    msg = unsafe_string(ccall((:rbffd_last_error, LIB), Cstring, ()))
+    code == 4 && throw(LinearAlgebra.SingularException(0))   # inv, src/interpolationmatrix.jl:8
+    code == 6 && throw(UndefRefError())                      # src/calculateneighbors.jl:8-41
+    (code == 1 || code == 2) && throw(ArgumentError(msg))
+    error("librbffd: " * msg)
+end
+
+# index sets -> concatenated Int64 arrays with 0-based offsets
+function _flatten(sets)
+    ptr = Int64[0]
+    idx = Int64[]
+    for s in sets
+        append!(idx, Int64.(collect(s)))
+        push!(ptr, length(idx))
+    end
+    return ptr, idx
+end
+
+function _operators(X::AbstractVector{<:SVector{2,Float64}}, Y, p, n, polydeg,
+                    ops::Vector{Int32}, k::Integer, sets)
+    N = length(X)
+    M = Y === nothing ? N : length(Y)
+    nnz = M * n
+    colptr = Vector{Int64}(undef, N + 1)
+    rowval = Vector{Int64}(undef, nnz)
+    nz = [Vector{Float64}(undef, nnz) for _ in ops]
+    opk = fill(Int32(k), length(ops))
+    ncols = Ref{Int64}(0)
+    stats = Ref{Stats}()
+    Xc = Vector{SVector{2,Float64}}(X)
+    Yc = Y === nothing ? nothing : Vector{SVector{2,Float64}}(Y)
+    in_idx = Int64[]; bc_ptr = Int64[]; bc_idx = Int64[]; g_ptr = Int64[]; g_idx = Int64[]
+    if sets !== nothing
+        X_idx_in, X_idx_bc, X_idx_bc_g = sets
+        in_idx = Int64.(collect(X_idx_in))
+        bc_ptr, bc_idx = _flatten(X_idx_bc)
+        g_ptr, g_idx = _flatten(X_idx_bc_g)
+    end
+    code = GC.@preserve Xc Yc colptr rowval nz opk ops in_idx bc_ptr bc_idx g_ptr g_idx begin
+        xp = Ptr{Float64}(pointer(Xc))
+        yp = Yc === nothing ? Ptr{Float64}(C_NULL) : Ptr{Float64}(pointer(Yc))
+        bnd = sets === nothing ? nothing :
+              Ref(Boundary(Int32(1), length(in_idx), pointer(in_idx), Int32(length(bc_ptr) - 1),
+                           pointer(bc_ptr), pointer(bc_idx), pointer(g_ptr), pointer(g_idx)))
+        nzp = [pointer(v) for v in nz]
+        GC.@preserve bnd nzp ccall((:rbffd_generate_operator, LIB), Cint,
+              (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Int32, Int32, Int32,
+               Ptr{Boundary}, Int32, Ptr{Int32}, Ptr{Int32}, Int32, Int32,
+               Ptr{Int64}, Ptr{Int64}, Ptr{Ptr{Float64}}, Ref{Int64}, Ref{Stats}),
+              context(), xp, N, yp, M, p, n, polydeg,
+              bnd === nothing ? Ptr{Boundary}(C_NULL) : Base.unsafe_convert(Ptr{Boundary}, bnd),
+              length(ops), ops, opk, PREC_FP64, Int32(1),
+              colptr, rowval, nzp, ncols, stats)
+    end
+    check(code)
+    nc = ncols[]
+    cp = colptr[1:nc+1]
+    return Tuple(SparseMatrixCSC(M, nc, cp, rowval, v) for v in nz)
+end
+
+const SIX = Int32[OP_E, OP_DX, OP_DY, OP_DXX, OP_DYY, OP_DXY]
+
+generate_operator(X, Y, p, n, polydeg) = _operators(X, Y, p, n, polydeg, SIX, 0, nothing)
+
+generate_operator(X, Y, p, n, polydeg, X_idx_in, X_idx_bc, X_idx_bc_g, Y_idx_in, Y_idx_bc, Y_idx_bc_g) =
+    _operators(X, Y, p, n, polydeg, SIX, 0, (X_idx_in, X_idx_bc, X_idx_bc_g))
+
+generate_operator(X, p, n, polydeg) = _operators(X, nothing, p, n, polydeg, SIX, 0, nothing)
+
+hyperviscosity_operator(k_deriv, X, Y, p, n, polydeg) =
+    _operators(X, Y, p, n, polydeg, Int32[OP_XK, OP_YK], k_deriv, nothing)
+
+hyperviscosity_operator(k_deriv, X, Y, p, n, polydeg, X_idx_in, X_idx_bc, X_idx_bc_g,
+                        Y_idx_in, Y_idx_bc, Y_idx_bc_g) =
+    _operators(X, Y, p, n, polydeg, Int32[OP_XK, OP_YK], k_deriv, (X_idx_in, X_idx_bc, X_idx_bc_g))
+
+hyperviscosity_operator(k_deriv, X, p, n, polydeg) =
+    _operators(X, nothing, p, n, polydeg, Int32[OP_XK, OP_YK], k_deriv, nothing)
+
+end # module
