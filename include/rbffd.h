@@ -124,8 +124,9 @@ int rbffd_device_info(rbffd_context *ctx, char *name, int32_t name_len, int32_t 
  * squared distances (may be NULL).  Order: ascending dx*dx + dy*dy, ties by the
  * lower index.  xgroup/qgroup (both or neither, may be NULL): a query with
  * qgroup < 0 sees all X nodes, one with qgroup = g >= 0 sees nodes with xgroup 0
- * or g.  Synchronises the stream (grid size is read back).  RBFFD_ERR_ARG if a
- * query sees fewer than k nodes. */
+ * or g.  Synchronises the stream (the failure flags are read back).
+ * RBFFD_ERR_ARG if a query sees fewer than k nodes; RBFFD_ERR_NONFINITE for
+ * non-finite points. */
 int rbffd_knn_device(rbffd_context *ctx, const double *X, int64_t N, const double *Q, int64_t M,
                      int32_t k, const int32_t *xgroup, const int32_t *qgroup, int32_t *idx, double *d2);
 
@@ -183,6 +184,42 @@ int rbffd_csr_to_csc_device(rbffd_context *ctx, int64_t M, int32_t n, int64_t nc
 int rbffd_spmv_device(rbffd_context *ctx, int64_t M, const int64_t *rowptr, const int32_t *col,
                       const void *vals, int32_t vals_type, int32_t F, const double *x, double *y,
                       double alpha, double beta, int32_t uniform_n);
+
+/* ------------------------------------------------------ uneven batch
+ * Many independent node sets, each with its own stencil size, polynomial
+ * degree and PHS power, generated in one call (Y = X for every set; method 1,
+ * or method 2 where xgroup/qgroup are given).  No reference counterpart as one
+ * call: it batches what a caller does with one generate_operator /
+ * hyperviscosity_operator per set (src/generate_operator.jl:29,192;
+ * src/hyperviscosity_operator.jl:26,177).  Every pointer in a set is a device
+ * pointer; outputs are the CSR rows of rbffd_weights_device. */
+typedef struct rbffd_batch_set {
+    const double *X;                /* N x 2 points */
+    int64_t N;
+    int32_t p, n, polydeg;          /* PHS power, stencil size, polynomial degree */
+    const int32_t *xgroup, *qgroup; /* N each, or NULL (see rbffd_boundary_groups) */
+    int32_t *colidx;                /* N x n, columns sorted per row */
+    double *vals[RBFFD_MAX_OPS];    /* N x n per operator */
+    int32_t *status;                /* N, or NULL */
+} rbffd_batch_set;
+
+/* Host: the group ids of method 2 for N nodes (xgroup, qgroup: N each), the
+ * visibility rule of calculateneighbors (src/calculateneighbors.jl:16-42,83-94).
+ * RBFFD_ERR_UNDEFINED if a node is in no index set. */
+int rbffd_boundary_groups(const rbffd_boundary *bnd, int64_t N, int32_t *xgroup, int32_t *qgroup);
+
+/* Generate every set.  The sets are spread over parallel branches (kNN and
+ * solves per set).  use_graph != 0: the first call with a given argument list
+ * runs eagerly (the layout autotuner runs there) and records the whole batch
+ * as a hipGraph; later calls with identical arguments (same pointers and
+ * sizes) replay the graph.  Synchronises.  set_status (host, nsets, may be
+ * NULL): 0 ok, 1 singular system, 2 non-finite input or weights, 3 a query sees
+ * fewer than n nodes.  Returns the error of the first failing set (outputs of
+ * the other sets are valid).  stats: rows, singular, nonfinite, and ms_solve =
+ * device time of the whole batch. */
+int rbffd_generate_batch_device(rbffd_context *ctx, int32_t nsets, const rbffd_batch_set *sets, int32_t nops,
+                                const int32_t *ops, const int32_t *opk, int32_t use_graph, int32_t *set_status,
+                                rbffd_stats *stats);
 
 /* ------------------------------------------------ host entry point (Julia)
  * generate_operator / hyperviscosity_operator with host arrays, returning the

@@ -17,16 +17,42 @@
 
 using rbffd::Error;
 
+struct rbffd_buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// Workspace slots of one neighbour search and the per-set buffers of a batch.
+enum ArenaSlot {
+    A_PART, A_BOX, A_GRID, A_COUNTS, A_START, A_CELL_OF, A_FILL, A_CELL_XY, A_CELL_IDX, A_CELL_GRP,
+    A_SCAN_TMP, A_STENCILS, A_STATUS, A_NSLOT
+};
+
+struct rbffd_arena {
+    rbffd_buf b[A_NSLOT];
+};
+
+constexpr int kLanes = 4;  // parallel branches of a captured batch
+
 struct rbffd_context {
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
-    struct Buf { void *p = nullptr; size_t bytes = 0; };
+    using Buf = rbffd_buf;
     Buf bufs[32];
     hipEvent_t ev[8] = {};
     hipEvent_t tune_ev[2] = {};
     // autotuner: (n, P, right-hand sides) -> lanes per stencil of the FP64 kernel
     std::map<std::tuple<int, int, int>, int> lu_group;
+    // neighbour-search workspace of the single-set calls, and one per batch branch
+    rbffd_arena knn_arena;
+    rbffd_arena lane_arena[kLanes];
+    hipStream_t lane_stream[kLanes] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kLanes] = {};
+    // the last captured batch: its arguments and executable graph
+    std::vector<int64_t> batch_key;
+    hipGraph_t batch_graph = nullptr;
+    hipGraphExec_t batch_exec = nullptr;
 };
 
 namespace {
@@ -37,7 +63,7 @@ enum Slot {
     S_BBOX_PART, S_BBOX, S_COUNTS, S_START, S_CELL_OF, S_FILL, S_CELL_XY, S_CELL_IDX, S_CELL_GRP,
     S_SCAN_TMP, S_FLAG, S_XGROUP, S_QGROUP, S_STENCILS, S_Y2X, S_STATUS, S_ROWS, S_CSC_COUNTS,
     S_CSC_PTR, S_CSC_FILL, S_CSC_PERM, S_CSC_ROW, S_H_X, S_H_Y, S_H_COL, S_H_VALS, S_H_CVALS,
-    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2
+    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS
 };
 
 template <class F>
@@ -58,11 +84,13 @@ int guarded(F &&f)
     }
 }
 
-void *ws(rbffd_context *c, int slot, size_t bytes)
+// Grow-only device buffer.  With may_alloc false (while a graph is being
+// captured) a buffer that is too small is an error instead of an allocation.
+void *grow(rbffd_buf &b, size_t bytes, bool may_alloc = true)
 {
-    auto &b = c->bufs[slot];
     if (bytes == 0) bytes = 16;
     if (b.bytes < bytes) {
+        if (!may_alloc) throw Error{RBFFD_ERR_RUNTIME, "batch workspace changed between runs"};
         if (b.p) RBFFD_HIP(hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
@@ -72,6 +100,8 @@ void *ws(rbffd_context *c, int slot, size_t bytes)
     }
     return b.p;
 }
+
+void *ws(rbffd_context *c, int slot, size_t bytes) { return grow(c->bufs[slot], bytes); }
 
 void check_ctx(rbffd_context *c)
 {
@@ -118,6 +148,43 @@ int read_flag(rbffd_context *c, const int *dflag)
 }
 
 // ------------------------------------------------------------ kNN
+// Enqueue one neighbour search on stream s: bounding box, grid parameters
+// (on the device, so nothing is read back), counting sort of X by cell, then
+// the query kernel.  flag[0] |= 1 when X is not finite; flag[1] counts
+// result slots left at -1 (a query seeing fewer than k nodes).  Neither
+// synchronises nor (with may_alloc false) allocates, so it can be captured.
+void knn_enqueue(rbffd_arena &A, bool may_alloc, const double *X, long N, const double *Q, long M, int k,
+                 const int *xgroup, const int *qgroup, int *idx, double *d2, int *flag, hipStream_t s)
+{
+    const int nb = (int)std::min<long>(1024, (N + 255) / 256);
+    double *part = (double *)grow(A.b[A_PART], sizeof(double) * 5 * nb, may_alloc);
+    double *box = (double *)grow(A.b[A_BOX], sizeof(double) * 5, may_alloc);
+    auto *grid = (rbffd::GridParams *)grow(A.b[A_GRID], sizeof(rbffd::GridParams), may_alloc);
+    const long ncell = rbffd::grid_max_cells(N);
+    int *counts = (int *)grow(A.b[A_COUNTS], sizeof(int) * (ncell + 1), may_alloc);
+    int *start = (int *)grow(A.b[A_START], sizeof(int) * (ncell + 1), may_alloc);
+    int *cell_of = (int *)grow(A.b[A_CELL_OF], sizeof(int) * N, may_alloc);
+    int *fill = (int *)grow(A.b[A_FILL], sizeof(int) * ncell, may_alloc);
+    double *cxy = (double *)grow(A.b[A_CELL_XY], sizeof(double) * 2 * N, may_alloc);
+    int *cidx = (int *)grow(A.b[A_CELL_IDX], sizeof(int) * N, may_alloc);
+    int *cgrp = xgroup ? (int *)grow(A.b[A_CELL_GRP], sizeof(int) * N, may_alloc) : nullptr;
+    const size_t tb = rbffd::scan_tmp_bytes(ncell + 1);
+    void *tmp = grow(A.b[A_SCAN_TMP], tb, may_alloc);
+
+    rbffd::launch_bbox(X, N, box, part, nb, s);
+    rbffd::launch_grid_params(box, N, k, grid, flag, s);
+    rbffd::launch_fill_i32(counts, ncell + 1, 0, s);
+    rbffd::launch_fill_i32(fill, ncell, 0, s);
+    rbffd::launch_cell_count(X, N, grid, cell_of, counts, s);
+    rbffd::launch_exclusive_scan(counts, start, ncell + 1, tmp, tb, s);
+    rbffd::launch_cell_scatter(X, xgroup, N, cell_of, start, fill, cxy, cidx, cgrp, s);
+    // X queries run in cell order so a wavefront walks neighbouring cells
+    const int *qorder = (Q == X && M == N) ? cidx : nullptr;
+    rbffd::launch_knn_query(grid, start, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    count_negative_kernel<<<nblk(M * (long)k), 256, 0, s>>>(idx, M * (long)k, flag + 1);
+    RBFFD_HIP(hipGetLastError());
+}
+
 void knn_impl(rbffd_context *c, const double *X, long N, const double *Q, long M, int k,
               const int *xgroup, const int *qgroup, int *idx, double *d2)
 {
@@ -127,61 +194,14 @@ void knn_impl(rbffd_context *c, const double *X, long N, const double *Q, long M
         throw Error{RBFFD_ERR_ARG, "knn: xgroup and qgroup go together"};
     if (M <= 0) return;
     hipStream_t s = c->stream;
-    const int nb = (int)std::min<long>(1024, (N + 255) / 256);
-    double *part = (double *)ws(c, S_BBOX_PART, sizeof(double) * 5 * nb);
-    double *box = (double *)ws(c, S_BBOX, sizeof(double) * 5);
-    rbffd::launch_bbox(X, N, box, part, nb, s);
-    double hb[5];
-    RBFFD_HIP(hipMemcpyAsync(hb, box, sizeof(hb), hipMemcpyDeviceToHost, s));
-    RBFFD_HIP(hipStreamSynchronize(s));
-    if (hb[4] != 0.0) throw Error{RBFFD_ERR_NONFINITE, "knn: non-finite coordinates"};
-
-    const double w = hb[2] - hb[0], h = hb[3] - hb[1];
-    const double ppc = std::max(2.0, k / 6.0);  // points per cell
-    rbffd::GridParams g{};
-    g.x0 = hb[0];
-    g.y0 = hb[1];
-    double cs;
-    if (w > 0 && h > 0) cs = sqrt(w * h * ppc / (double)N);
-    else cs = std::max(w, h) * ppc / (double)N;
-    if (!(cs > 0)) cs = 1.0;
-    long gx, gy;
-    for (;;) {
-        gx = w > 0 ? std::min<long>((long)(w / cs) + 1, 1L << 16) : 1;
-        gy = h > 0 ? std::min<long>((long)(h / cs) + 1, 1L << 16) : 1;
-        if (gx * gy <= 4 * N + 64) break;
-        cs *= 1.25;
-    }
-    g.gx = (int)gx;
-    g.gy = (int)gy;
-    g.csx = w > 0 ? w / (double)gx * (1.0 + 1e-12) : 1.0;
-    g.csy = h > 0 ? h / (double)gy * (1.0 + 1e-12) : 1.0;
-    const long ncell = gx * gy;
-
-    int *counts = (int *)ws(c, S_COUNTS, sizeof(int) * (ncell + 1));
-    int *start = (int *)ws(c, S_START, sizeof(int) * (ncell + 1));
-    int *cell_of = (int *)ws(c, S_CELL_OF, sizeof(int) * N);
-    int *fill = (int *)ws(c, S_FILL, sizeof(int) * ncell);
-    double *cxy = (double *)ws(c, S_CELL_XY, sizeof(double) * 2 * N);
-    int *cidx = (int *)ws(c, S_CELL_IDX, sizeof(int) * N);
-    int *cgrp = xgroup ? (int *)ws(c, S_CELL_GRP, sizeof(int) * N) : nullptr;
-    RBFFD_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (ncell + 1), s));
-    RBFFD_HIP(hipMemsetAsync(fill, 0, sizeof(int) * ncell, s));
-    rbffd::launch_cell_count(X, N, g, cell_of, counts, s);
-    size_t tb = rbffd::scan_tmp_bytes(ncell + 1);
-    void *tmp = ws(c, S_SCAN_TMP, tb);
-    rbffd::launch_exclusive_scan(counts, start, ncell + 1, tmp, tb, s);
-    rbffd::launch_cell_scatter(X, xgroup, N, cell_of, start, fill, cxy, cidx, cgrp, s);
-    // X queries run in cell order so a wavefront walks neighbouring cells
-    const int *qorder = (Q == X && M == N) ? cidx : nullptr;
-    rbffd::launch_knn_query(g, start, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    RBFFD_HIP(hipGetLastError());
-
     int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
-    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
-    count_negative_kernel<<<nblk(M * (long)k), 256, 0, s>>>(idx, M * (long)k, flag);
-    if (read_flag(c, flag) != 0)
-        throw Error{RBFFD_ERR_ARG, "knn: a query sees fewer than k nodes"};
+    rbffd::launch_fill_i32(flag, 2, 0, s);
+    knn_enqueue(c->knn_arena, true, X, N, Q, M, k, xgroup, qgroup, idx, d2, flag, s);
+    int h[2] = {0, 0};
+    RBFFD_HIP(hipMemcpyAsync(h, flag, sizeof(h), hipMemcpyDeviceToHost, s));
+    RBFFD_HIP(hipStreamSynchronize(s));
+    if (h[0] & 1) throw Error{RBFFD_ERR_NONFINITE, "knn: non-finite coordinates"};
+    if (h[1] != 0) throw Error{RBFFD_ERR_ARG, "knn: a query sees fewer than k nodes"};
 }
 
 int n_monomials(int polydeg) { return (polydeg + 1) * (polydeg + 2) / 2; }
@@ -201,7 +221,7 @@ void check_ops(int nops, const int *ops, const int *opk)
 // for a shape, the first call with that shape on a context times each on a
 // sample of the batch (the sample's outputs are rewritten by the real launch)
 // and keeps the fastest.  RBFFD_LU_GROUP=<lanes> forces a layout.
-int choose_group(rbffd_context *c, const rbffd::SolveArgs &a)
+int choose_group(rbffd_context *c, const rbffd::SolveArgs &a, bool may_tune = true)
 {
     int groups[rbffd::kMaxGroups];
     rbffd::solve_groups(a.n, a.P, 0, groups);
@@ -217,7 +237,7 @@ int choose_group(rbffd_context *c, const rbffd::SolveArgs &a)
     auto key = std::make_tuple(a.n, a.P, rb);
     auto it = c->lu_group.find(key);
     if (it != c->lu_group.end()) return it->second;
-    if (a.ntasks < 4096) return groups[0];  // too small to time; keep the default
+    if (a.ntasks < 4096 || !may_tune) return groups[0];  // too small to time (or capturing): the default
     for (auto &e : c->tune_ev)
         if (!e) RBFFD_HIP(hipEventCreate(&e));
     rbffd::SolveArgs t = a;
@@ -241,12 +261,9 @@ int choose_group(rbffd_context *c, const rbffd::SolveArgs &a)
     return pick;
 }
 
-void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, long M, const int *stencils,
-                  int n, const int *y2x, const int *rows, long nrows, int p, int polydeg, int method,
-                  int nops, const int *ops, const int *opk, int precision, int *colidx,
-                  double *const *vals, int *status)
+void check_weights_args(int n, int p, int polydeg, int method, int nops, const int *ops, const int *opk,
+                        int precision)
 {
-    (void)N;
     if (method < 1 || method > 3) throw Error{RBFFD_ERR_ARG, "method must be 1, 2 or 3"};
     if (p < 1) throw Error{RBFFD_ERR_ARG, "PHS power must be >= 1"};
     if (polydeg < 0 || polydeg > 8) throw Error{RBFFD_ERR_ARG, "polynomial degree must be in 0..8"};
@@ -258,6 +275,16 @@ void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, lo
         throw Error{RBFFD_ERR_UNSUPPORTED, "no compiled kernel for n=" + std::to_string(n) +
                                                ", polydeg=" + std::to_string(polydeg) +
                                                (precision ? " (mixed precision)" : "")};
+}
+
+// Enqueue the local solves on stream s (the autotuner, when allowed, times on
+// the context stream).
+void solves_enqueue(rbffd_context *c, const double *X, const double *Y, long M, const int *stencils, int n,
+                    const int *y2x, const int *rows, long nrows, int p, int polydeg, int method, int nops,
+                    const int *ops, const int *opk, int precision, int *colidx, double *const *vals,
+                    int *status, hipStream_t s, bool may_tune)
+{
+    const int P = n_monomials(polydeg);
     // at most kMaxRhs right-hand sides per launch; more operators run in chunks
     for (int o0 = 0; o0 < nops; o0 += rbffd::kMaxRhs) {
         const int cnt = std::min(rbffd::kMaxRhs, nops - o0);
@@ -283,10 +310,21 @@ void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, lo
         a.status_accumulate = o0 > 0;
         a.max_iter = 8;
         a.tol = 4e-16;
-        a.group = precision == RBFFD_PREC_FP64 ? choose_group(c, a) : 0;
-        rbffd::launch_solve(a, precision, c->stream);
+        a.group = precision == RBFFD_PREC_FP64 ? choose_group(c, a, may_tune) : 0;
+        rbffd::launch_solve(a, precision, s);
         RBFFD_HIP(hipGetLastError());
     }
+}
+
+void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, long M, const int *stencils,
+                  int n, const int *y2x, const int *rows, long nrows, int p, int polydeg, int method,
+                  int nops, const int *ops, const int *opk, int precision, int *colidx,
+                  double *const *vals, int *status)
+{
+    (void)N;
+    check_weights_args(n, p, polydeg, method, nops, ops, opk, precision);
+    solves_enqueue(c, X, Y, M, stencils, n, y2x, rows, nrows, p, polydeg, method, nops, ops, opk, precision,
+                   colidx, vals, status, c->stream, true);
 }
 
 // group ids that restate calculateneighbors' masks (src/calculateneighbors.jl:16-42, 83-87)
@@ -445,6 +483,161 @@ void csc_impl(rbffd_context *c, long M, int n, long ncols, const int *colidx, in
     RBFFD_HIP(hipStreamSynchronize(s));
 }
 
+// ------------------------------------------------------------ batch
+// Per set: 8 flag words.  [0] bit 0: X not finite; [1] kNN result slots at -1;
+// [2 + s] rows with status s (1 singular, 2 non-finite weights).
+constexpr int kSetFlags = 8;
+
+void batch_enqueue(rbffd_context *c, const rbffd_batch_set *sets, int nsets, const std::vector<int> &lane_of,
+                   int nops, const int *ops, const int *opk, int *flags, bool capture)
+{
+    hipStream_t main = c->stream;
+    rbffd::launch_fill_i32(flags, (long)kSetFlags * nsets, 0, main);
+    if (capture) {
+        RBFFD_HIP(hipEventRecord(c->fork_ev, main));
+        for (int l = 0; l < kLanes; ++l) RBFFD_HIP(hipStreamWaitEvent(c->lane_stream[l], c->fork_ev, 0));
+    }
+    for (int i = 0; i < nsets; ++i) {
+        const rbffd_batch_set &S = sets[i];
+        const int l = lane_of[i];
+        hipStream_t s = capture ? c->lane_stream[l] : main;
+        rbffd_arena &A = c->lane_arena[l];
+        int *st = (int *)grow(A.b[A_STENCILS], sizeof(int) * (size_t)S.N * S.n, !capture);
+        int *status = S.status ? S.status : (int *)grow(A.b[A_STATUS], sizeof(int) * (size_t)S.N, !capture);
+        int *f = flags + (long)kSetFlags * i;
+        knn_enqueue(A, !capture, S.X, S.N, S.X, S.N, S.n, S.xgroup, S.qgroup, st, nullptr, f, s);
+        solves_enqueue(c, S.X, S.X, S.N, st, S.n, nullptr, nullptr, 0, S.p, S.polydeg, 1, nops, ops, opk,
+                       RBFFD_PREC_FP64, S.colidx, S.vals, status, s, !capture);
+        status_count_kernel<<<nblk(S.N), 256, 0, s>>>(status, S.N, f + 2);
+        RBFFD_HIP(hipGetLastError());
+    }
+    if (capture) {
+        for (int l = 0; l < kLanes; ++l) {
+            RBFFD_HIP(hipEventRecord(c->join_ev[l], c->lane_stream[l]));
+            RBFFD_HIP(hipStreamWaitEvent(main, c->join_ev[l], 0));
+        }
+    }
+}
+
+std::vector<int64_t> batch_key(const rbffd_batch_set *sets, int nsets, int nops, const int *ops, const int *opk)
+{
+    std::vector<int64_t> k{nsets, nops};
+    for (int o = 0; o < nops; ++o) {
+        k.push_back(ops[o]);
+        k.push_back(opk ? opk[o] : 0);
+    }
+    for (int i = 0; i < nsets; ++i) {
+        const auto &S = sets[i];
+        for (int64_t v : {(int64_t)(intptr_t)S.X, S.N, (int64_t)S.p, (int64_t)S.n, (int64_t)S.polydeg,
+                          (int64_t)(intptr_t)S.xgroup, (int64_t)(intptr_t)S.qgroup,
+                          (int64_t)(intptr_t)S.colidx, (int64_t)(intptr_t)S.status})
+            k.push_back(v);
+        for (int o = 0; o < nops; ++o) k.push_back((int64_t)(intptr_t)S.vals[o]);
+    }
+    return k;
+}
+
+void batch_impl(rbffd_context *c, int nsets, const rbffd_batch_set *sets, int nops, const int *ops,
+                const int *opk, int use_graph, int32_t *set_status, rbffd_stats *stats)
+{
+    check_ctx(c);
+    if (nsets < 1 || !sets) throw Error{RBFFD_ERR_ARG, "batch: no sets"};
+    check_ops(nops, ops, opk);
+    std::vector<double> work(nsets);
+    for (int i = 0; i < nsets; ++i) {
+        const auto &S = sets[i];
+        if (!S.X || !S.colidx) throw Error{RBFFD_ERR_ARG, "batch: null X or colidx in set " + std::to_string(i)};
+        for (int o = 0; o < nops; ++o)
+            if (!S.vals[o]) throw Error{RBFFD_ERR_ARG, "batch: null vals in set " + std::to_string(i)};
+        if (S.n < 1 || S.N < S.n) throw Error{RBFFD_ERR_ARG, "batch: set " + std::to_string(i) + " has fewer nodes than n"};
+        if ((S.xgroup == nullptr) != (S.qgroup == nullptr))
+            throw Error{RBFFD_ERR_ARG, "batch: xgroup and qgroup go together"};
+        check_weights_args(S.n, S.p, S.polydeg, 1, nops, ops, opk, RBFFD_PREC_FP64);
+        const double ns = S.n + n_monomials(S.polydeg);
+        work[i] = (double)S.N * ns * ns * ns;
+    }
+    // longest first onto the least loaded branch
+    std::vector<int> order(nsets), lane_of(nsets);
+    for (int i = 0; i < nsets; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    double load[kLanes] = {};
+    for (int i : order) {
+        int l = (int)(std::min_element(load, load + kLanes) - load);
+        lane_of[i] = l;
+        load[l] += work[i];
+    }
+    int *flags = (int *)ws(c, S_BATCH_FLAGS, sizeof(int) * kSetFlags * nsets);
+    for (int e = 0; e < 2; ++e)
+        if (!c->ev[e]) RBFFD_HIP(hipEventCreate(&c->ev[e]));
+    hipStream_t main = c->stream;
+    auto key = batch_key(sets, nsets, nops, ops, opk);
+    RBFFD_HIP(hipEventRecord(c->ev[0], main));
+    if (use_graph && c->batch_exec && key == c->batch_key) {
+        RBFFD_HIP(hipGraphLaunch(c->batch_exec, main));
+    } else {
+        batch_enqueue(c, sets, nsets, lane_of, nops, ops, opk, flags, false);  // also tunes the layouts
+        if (use_graph) {
+            if (c->batch_exec) { RBFFD_HIP(hipGraphExecDestroy(c->batch_exec)); c->batch_exec = nullptr; }
+            if (c->batch_graph) { RBFFD_HIP(hipGraphDestroy(c->batch_graph)); c->batch_graph = nullptr; }
+            c->batch_key.clear();
+            for (int l = 0; l < kLanes; ++l) {
+                if (!c->lane_stream[l]) RBFFD_HIP(hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking));
+                if (!c->join_ev[l]) RBFFD_HIP(hipEventCreateWithFlags(&c->join_ev[l], hipEventDisableTiming));
+            }
+            if (!c->fork_ev) RBFFD_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+            // the eager pass above must finish before capture: the graph reuses its buffers
+            RBFFD_HIP(hipStreamSynchronize(main));
+            RBFFD_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                batch_enqueue(c, sets, nsets, lane_of, nops, ops, opk, flags, true);
+            } catch (...) {
+                (void)hipStreamEndCapture(main, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            RBFFD_HIP(hipStreamEndCapture(main, &g));
+            c->batch_graph = g;
+            RBFFD_HIP(hipGraphInstantiate(&c->batch_exec, g, nullptr, nullptr, 0));
+            c->batch_key = key;
+        }
+    }
+    RBFFD_HIP(hipEventRecord(c->ev[1], main));
+    std::vector<int> hf((size_t)kSetFlags * nsets);
+    RBFFD_HIP(hipMemcpyAsync(hf.data(), flags, sizeof(int) * hf.size(), hipMemcpyDeviceToHost, main));
+    RBFFD_HIP(hipStreamSynchronize(main));
+    long nsing = 0, nnonf = 0, rows = 0;
+    int first_code = RBFFD_OK, nbad = 0;
+    for (int i = 0; i < nsets; ++i) {
+        const int *f = &hf[(size_t)kSetFlags * i];
+        int st = 0, code = RBFFD_OK;
+        if (f[0] & 1) { st = 2; code = RBFFD_ERR_NONFINITE; }
+        else if (f[1] != 0) { st = 3; code = RBFFD_ERR_ARG; }
+        else if (f[3] != 0) { st = 1; code = RBFFD_ERR_SINGULAR; }
+        else if (f[4] != 0) { st = 2; code = RBFFD_ERR_NONFINITE; }
+        if (set_status) set_status[i] = st;
+        nsing += f[3];
+        nnonf += f[4];
+        rows += sets[i].N;
+        if (code != RBFFD_OK) {
+            ++nbad;
+            if (first_code == RBFFD_OK) first_code = code;
+        }
+    }
+    if (stats) {
+        *stats = rbffd_stats{};
+        float ms = 0;
+        RBFFD_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        stats->rows = rows;
+        stats->singular = nsing;
+        stats->nonfinite = nnonf;
+        stats->ms_solve = ms;
+    }
+    if (first_code != RBFFD_OK)
+        throw Error{first_code, std::to_string(nbad) + " of " + std::to_string(nsets) +
+                                    " sets failed (see set_status)"};
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -476,12 +669,26 @@ int rbffd_destroy(rbffd_context *c)
         if (!c) return;
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        for (auto &st : c->lane_stream)
+            if (st) (void)hipStreamSynchronize(st);
+        if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
+        if (c->batch_graph) (void)hipGraphDestroy(c->batch_graph);
         for (auto &b : c->bufs)
             if (b.p) (void)hipFree(b.p);
+        for (auto &b : c->knn_arena.b)
+            if (b.p) (void)hipFree(b.p);
+        for (auto &A : c->lane_arena)
+            for (auto &b : A.b)
+                if (b.p) (void)hipFree(b.p);
         for (auto &e : c->ev)
             if (e) (void)hipEventDestroy(e);
         for (auto &e : c->tune_ev)
             if (e) (void)hipEventDestroy(e);
+        if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+        for (auto &e : c->join_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &st : c->lane_stream)
+            if (st) (void)hipStreamDestroy(st);
         if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
         delete c;
     });
@@ -704,6 +911,24 @@ int rbffd_generate_operator(rbffd_context *c, const double *X, int64_t N, const 
             RBFFD_HIP(hipMemcpyAsync(nzval[o], cvp[o], sizeof(double) * nnz, hipMemcpyDeviceToHost, s));
         RBFFD_HIP(hipStreamSynchronize(s));
     });
+}
+
+int rbffd_boundary_groups(const rbffd_boundary *bnd, int64_t N, int32_t *xgroup, int32_t *qgroup)
+{
+    return guarded([&] {
+        if (!bnd || !xgroup || !qgroup || N < 1) throw Error{RBFFD_ERR_ARG, "boundary_groups: bad arguments"};
+        std::vector<int> xg, qg;
+        boundary_groups(bnd, N, xg, qg);
+        std::copy(xg.begin(), xg.end(), xgroup);
+        std::copy(qg.begin(), qg.end(), qgroup);
+    });
+}
+
+int rbffd_generate_batch_device(rbffd_context *c, int32_t nsets, const rbffd_batch_set *sets, int32_t nops,
+                                const int32_t *ops, const int32_t *opk, int32_t use_graph, int32_t *set_status,
+                                rbffd_stats *stats)
+{
+    return guarded([&] { batch_impl(c, nsets, sets, nops, ops, opk, use_graph, set_status, stats); });
 }
 
 }  // extern "C"

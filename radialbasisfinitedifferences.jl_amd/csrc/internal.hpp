@@ -29,13 +29,21 @@ struct GridParams {
     double x0, y0;    // lower-left corner
     double csx, csy;  // cell sizes
     int gx, gy;       // cells per axis
+    int valid;        // 0: the points are not finite; queries return -1
 };
 
 // Bounding box of X (device): box[5] = (xmin, ymin, xmax, ymax, non-finite count);
 // partials holds 5 values per block.
 void launch_bbox(const double *X, long N, double *box, double *partials, int nblocks, hipStream_t s);
 // Cell index of every point and the per-cell counts (counts must be zeroed).
-void launch_cell_count(const double *X, long N, GridParams g, int *cell_of, int *counts, hipStream_t s);
+void launch_cell_count(const double *X, long N, const GridParams *g, int *cell_of, int *counts, hipStream_t s);
+// Upper bound on the grid's cell count for N points.
+__host__ __device__ inline long grid_max_cells(long N) { return 4 * N + 64; }
+// Grid parameters computed on the device from box (see launch_bbox); bit 0 of
+// *flag (may be NULL) is set when the box is not finite.
+void launch_grid_params(const double *box, long N, int k, GridParams *out, int *flag, hipStream_t s);
+// p[0 .. n) = v; a kernel, so it can be recorded into a graph.
+void launch_fill_i32(int *p, long n, int v, hipStream_t s);
 // Exclusive scan of counts (ncell) into start (ncell + 1); tmp as returned by scan_tmp_bytes.
 size_t scan_tmp_bytes(long n);
 void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tmp_bytes, hipStream_t s);
@@ -43,7 +51,7 @@ void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tm
 void launch_cell_scatter(const double *X, const int *xgroup, long N, const int *cell_of, const int *start,
                          int *fill, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s);
 // Exact kNN of every query, ascending (d^2, index).
-void launch_knn_query(GridParams g, const int *cell_start, const double *cell_xy, const int *cell_idx,
+void launch_knn_query(const GridParams *g, const int *cell_start, const double *cell_xy, const int *cell_idx,
                       const int *cell_grp, const double *Q, const int *qgroup, const int *qorder, long M,
                       int k, int *idx_out, double *d2_out, hipStream_t s);
 

@@ -82,10 +82,12 @@ __device__ __forceinline__ int cell_coord(double v, double v0, double cs, int g)
     return c >= g ? g - 1 : c;
 }
 
-__global__ void cell_count_kernel(const double *X, long N, GridParams g, int *cell_of, int *counts)
+__global__ void cell_count_kernel(const double *X, long N, const GridParams *__restrict__ gp, int *cell_of,
+                                  int *counts)
 {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= N) return;
+    const GridParams g = *gp;
     int cx = cell_coord(X[2 * i], g.x0, g.csx, g.gx);
     int cy = cell_coord(X[2 * i + 1], g.y0, g.csy, g.gy);
     int c = cy * g.gx + cx;
@@ -93,10 +95,65 @@ __global__ void cell_count_kernel(const double *X, long N, GridParams g, int *ce
     atomicAdd(&counts[c], 1);
 }
 
-void launch_cell_count(const double *X, long N, GridParams g, int *cell_of, int *counts, hipStream_t s)
+void launch_cell_count(const double *X, long N, const GridParams *gp, int *cell_of, int *counts, hipStream_t s)
 {
     if (N <= 0) return;
-    cell_count_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, N, g, cell_of, counts);
+    cell_count_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, N, gp, cell_of, counts);
+}
+
+// Grid of about ppc points per cell over the bounding box, at most
+// grid_max_cells(N) cells.  A box with non-finite entries (non-finite
+// coordinates) gets a one-cell grid, which every later kernel handles safely,
+// and sets bit 0 of *flag.
+__global__ void grid_params_kernel(const double *box, long N, int k, GridParams *out, int *flag)
+{
+    const double x0 = box[0], y0 = box[1], w = box[2] - box[0], h = box[3] - box[1];
+    GridParams g;
+    if (box[4] != 0.0 || !__builtin_isfinite(w) || !__builtin_isfinite(h) || !__builtin_isfinite(x0) ||
+        !__builtin_isfinite(y0)) {
+        g.x0 = 0.0; g.y0 = 0.0; g.csx = 1.0; g.csy = 1.0; g.gx = 1; g.gy = 1; g.valid = 0;
+        *out = g;
+        if (flag) atomicOr(flag, 1);
+        return;
+    }
+    const double ppc = fmax(2.0, k / 6.0);
+    double cs;
+    if (w > 0 && h > 0) cs = sqrt(w * h * ppc / (double)N);
+    else cs = fmax(w, h) * ppc / (double)N;
+    if (!(cs > 0)) cs = 1.0;
+    const long cap = grid_max_cells(N);
+    long gx, gy;
+    for (;;) {
+        gx = w > 0 ? min((long)(w / cs) + 1, 1L << 16) : 1;
+        gy = h > 0 ? min((long)(h / cs) + 1, 1L << 16) : 1;
+        if (gx * gy <= cap) break;
+        cs *= 1.25;
+    }
+    g.x0 = x0;
+    g.y0 = y0;
+    g.gx = (int)gx;
+    g.gy = (int)gy;
+    g.csx = w > 0 ? w / (double)gx * (1.0 + 1e-12) : 1.0;
+    g.csy = h > 0 ? h / (double)gy * (1.0 + 1e-12) : 1.0;
+    g.valid = 1;
+    *out = g;
+}
+
+void launch_grid_params(const double *box, long N, int k, GridParams *out, int *flag, hipStream_t s)
+{
+    grid_params_kernel<<<1, 1, 0, s>>>(box, N, k, out, flag);
+}
+
+__global__ void fill_i32_kernel(int *p, long n, int v)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_i32(int *p, long n, int v, hipStream_t s)
+{
+    if (n <= 0) return;
+    fill_i32_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
 }
 
 size_t scan_tmp_bytes(long n)
@@ -142,7 +199,8 @@ __device__ __forceinline__ bool key_less(double d, int i, double d2, int i2)
 // Slots [0, KMAX-k) hold a sentinel smaller than any key, so the k real
 // entries always end at slot KMAX-1 and every register index is static.
 template <int KMAX>
-__global__ __launch_bounds__(256) void knn_query_kernel(GridParams g, const int *__restrict__ cell_start,
+__global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__restrict__ gp,
+                                                        const int *__restrict__ cell_start,
                                                         const double *__restrict__ cell_xy,
                                                         const int *__restrict__ cell_idx,
                                                         const int *__restrict__ cell_grp,
@@ -154,7 +212,15 @@ __global__ __launch_bounds__(256) void knn_query_kernel(GridParams g, const int 
 #pragma clang fp contract(off)
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= M) return;
+    const GridParams g = *gp;
     const long q = qorder ? (long)qorder[t] : t;
+    if (!g.valid) {  // non-finite points: no search, every slot -1
+        for (int j = 0; j < k; ++j) {
+            idx_out[q * k + j] = -1;
+            if (d2_out) d2_out[q * k + j] = INFINITY;
+        }
+        return;
+    }
     const double qx = Q[2 * q], qy = Q[2 * q + 1];
     const int qg = (cell_grp && qgroup) ? qgroup[q] : -1;
 
@@ -241,7 +307,7 @@ __global__ __launch_bounds__(256) void knn_query_kernel(GridParams g, const int 
 }
 
 template <int KMAX>
-static void knn_launch_t(GridParams g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
+static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
                          const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
                          double *d2, hipStream_t s)
 {
@@ -249,7 +315,7 @@ static void knn_launch_t(GridParams g, const int *cs, const double *cxy, const i
                                                                          qorder, M, k, idx, d2);
 }
 
-void launch_knn_query(GridParams g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
+void launch_knn_query(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
                       const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
                       double *d2, hipStream_t s)
 {

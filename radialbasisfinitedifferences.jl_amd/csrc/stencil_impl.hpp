@@ -75,7 +75,8 @@ __global__ __launch_bounds__(64 * WPB) void solve_kernel(const SolveArgs a)
 
     const long yk = a.rows ? (long)a.rows[task] : task;
     const long xs = a.y2x ? (long)a.y2x[yk] : yk;
-    const int gidx = lane < NN ? a.stencils[xs * NN + lane] : 0;
+    // a failed neighbour search leaves -1 entries; clamp so every read stays in X
+    const int gidx = lane < NN ? max(a.stencils[xs * NN + lane], 0) : 0;
     const int c = __builtin_amdgcn_readfirstlane(gidx);
 
     // ---- stencil frame (src/scalestencil.jl:10-21, src/generate_operator.jl:100-134)

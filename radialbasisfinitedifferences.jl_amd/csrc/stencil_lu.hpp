@@ -222,7 +222,8 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     // ---- stencil frame (src/scalestencil.jl:10-21, src/generate_operator.jl:100-134)
     int gid[NODES];
     double dxn[NODES], dyn[NODES];
-    const int c = a.stencils[xs * NN];
+    // a failed neighbour search leaves -1 entries; clamp so every read stays in X
+    const int c = max(a.stencils[xs * NN], 0);
     double sx = 1.0, sy = 1.0, yx, yy;
     {
 #pragma clang fp contract(off)
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
 #pragma unroll
         for (int r = 0; r < NODES; ++r) {
             const int i = t + G * r;
-            gid[r] = i < NN ? a.stencils[xs * NN + i] : c;
+            gid[r] = i < NN ? max(a.stencils[xs * NN + i], 0) : c;
             dxn[r] = a.X[2 * (long)gid[r]] - cx;
             dyn[r] = a.X[2 * (long)gid[r] + 1] - cy;
             mx = fmax(mx, __builtin_fabs(dxn[r]));

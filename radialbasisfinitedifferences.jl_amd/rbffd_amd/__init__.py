@@ -2,7 +2,8 @@
 on librbffd, a gfx950 (MI355X) HIP engine behind a C ABI (include/rbffd.h)."""
 from ._lib import NonFiniteError, RbffdError, SingularException, UndefRefError, load
 from .device import Context, DeviceArray
-from .operators import Engine, default_context, generate_operator, hyperviscosity_operator, operators
+from .operators import (Engine, boundary_groups, default_context, generate_operator, hyperviscosity_operator,
+                        operators)
 
-__all__ = ["Context", "DeviceArray", "Engine", "generate_operator", "hyperviscosity_operator", "operators",
+__all__ = ["Context", "DeviceArray", "Engine", "boundary_groups", "generate_operator", "hyperviscosity_operator", "operators",
            "default_context", "load", "SingularException", "UndefRefError", "NonFiniteError", "RbffdError"]

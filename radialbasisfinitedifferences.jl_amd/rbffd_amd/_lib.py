@@ -70,6 +70,21 @@ class rbffd_stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class rbffd_batch_set(ctypes.Structure):
+    _fields_ = [
+        ("X", ctypes.c_void_p),
+        ("N", ctypes.c_int64),
+        ("p", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("polydeg", ctypes.c_int32),
+        ("xgroup", ctypes.c_void_p),
+        ("qgroup", ctypes.c_void_p),
+        ("colidx", ctypes.c_void_p),
+        ("vals", ctypes.c_void_p * 10),
+        ("status", ctypes.c_void_p),
+    ]
+
+
 _V = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -101,6 +116,9 @@ SIGNATURES = {
     "rbffd_csr_to_csc_device": (ctypes.c_int, [_V, _I64, _I32, _I64, _V, _I32, _V, _V, _V, _V, _I32,
                                                ctypes.POINTER(_I64)]),
     "rbffd_spmv_device": (ctypes.c_int, [_V, _I64, _V, _V, _V, _I32, _I32, _V, _V, _D, _D, _I32]),
+    "rbffd_boundary_groups": (ctypes.c_int, [ctypes.POINTER(rbffd_boundary), _I64, _V, _V]),
+    "rbffd_generate_batch_device": (ctypes.c_int, [_V, _I32, ctypes.POINTER(rbffd_batch_set), _I32, _V, _V, _I32,
+                                                   _V, ctypes.POINTER(rbffd_stats)]),
     "rbffd_generate_operator": (ctypes.c_int, [_V, _V, _I64, _V, _I64, _I32, _I32, _I32,
                                                ctypes.POINTER(rbffd_boundary), _I32, _V, _V, _I32, _I32, _V, _V,
                                                _V, ctypes.POINTER(_I64), ctypes.POINTER(rbffd_stats)]),

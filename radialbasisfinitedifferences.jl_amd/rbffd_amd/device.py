@@ -99,6 +99,10 @@ class DeviceArray:
             raise ValueError("size mismatch")
         check(_lib.load().rbffd_memcpy_d2d(self.ctx.handle, self._p, other.ptr, self.nbytes))
 
+    def zero_(self):
+        check(_lib.load().rbffd_memset_device(self.ctx.handle, self._p, 0, self.nbytes))
+        return self
+
     def get(self):
         out = np.empty(self.shape, self.dtype)
         check(_lib.load().rbffd_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p), self._p,

@@ -27,7 +27,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib
-from ._lib import OP, PREC, check, rbffd_boundary, rbffd_stats
+from ._lib import OP, PREC, check, rbffd_batch_set, rbffd_boundary, rbffd_stats
 from .device import Context, DeviceArray
 
 _default_ctx = None
@@ -68,6 +68,17 @@ class _Boundary:
         self.gh_idx = np.ascontiguousarray(np.concatenate(ghs) if ghs else np.zeros(0, np.int64))
         self.s = rbffd_boundary(index_base, len(self.in_idx), _vp(self.in_idx), len(bcs), _vp(self.bc_ptr),
                                 _vp(self.bc_idx), _vp(self.gh_ptr), _vp(self.gh_idx))
+
+
+def boundary_groups(N, X_idx_in, X_idx_bc, X_idx_bc_g, index_base=0):
+    """Method-2 group ids (xgroup, qgroup) of N nodes: the visibility rule of
+    calculateneighbors (src/calculateneighbors.jl:16-42, 83-94)."""
+    L = _lib.load()
+    bnd = _Boundary(X_idx_in, X_idx_bc, X_idx_bc_g, index_base)
+    xg = np.empty(N, np.int32)
+    qg = np.empty(N, np.int32)
+    check(L.rbffd_boundary_groups(ctypes.byref(bnd.s), int(N), _vp(xg), _vp(qg)))
+    return xg, qg
 
 
 def _op_arrays(ops, k):
@@ -212,6 +223,41 @@ class Engine:
                                             y2x_out.ptr if y2x_out is not None else None, ctypes.byref(stats))
         check(code)
         return colidx, vals, stats.as_dict()
+
+    def generate_batch(self, sets, ops, k=0, use_graph=True):
+        """Uneven batch: sets is a list of dicts with X (DeviceArray, N x 2), p,
+        n, polydeg and optionally xgroup/qgroup (DeviceArray int32, method-2
+        groups from boundary_groups) and colidx/vals/status outputs (allocated
+        here when absent).  Returns (per-set [(colidx, vals, status)], set
+        status array, stats); raises on the first failing set like the
+        single-set calls."""
+        kinds, ks = _op_arrays(ops, k)
+        arr = (rbffd_batch_set * len(sets))()
+        out = []
+        for i, S in enumerate(sets):
+            X = S["X"]
+            N, n = X.shape[0], int(S["n"])
+            col = S["colidx"] if S.get("colidx") is not None else self.ctx.empty((N, n), np.int32)
+            vals = S["vals"] if S.get("vals") is not None else [self.ctx.empty((N, n), np.float64) for _ in ops]
+            st = S["status"] if S.get("status") is not None else self.ctx.empty((N,), np.int32)
+            b = arr[i]
+            b.X, b.N, b.p, b.n, b.polydeg = X.ptr, N, int(S["p"]), n, int(S["polydeg"])
+            b.xgroup = S["xgroup"].ptr if S.get("xgroup") is not None else None
+            b.qgroup = S["qgroup"].ptr if S.get("qgroup") is not None else None
+            b.colidx = col.ptr
+            for o, v in enumerate(vals):
+                b.vals[o] = v.ptr
+            b.status = st.ptr
+            out.append((col, vals, st))
+        set_status = np.zeros(len(sets), np.int32)
+        stats = rbffd_stats()
+        self._batch_keepalive = (arr, out)
+        code = self.L.rbffd_generate_batch_device(self.ctx.handle, len(sets), arr, len(ops), _vp(kinds), _vp(ks),
+                                                  1 if use_graph else 0, _vp(set_status), ctypes.byref(stats))
+        if code != 0:
+            self.last_set_status = set_status
+        check(code)
+        return out, set_status, stats.as_dict()
 
     def csr_to_csc(self, colidx, vals, ncols, index_base=0):
         M, n = colidx.shape
