@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-e2e", action="store_true", help="skip the host-level end-to-end leg")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other workloads (hyperviscosity 4M, uneven batch, time stepping)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1"}))
@@ -164,18 +166,19 @@ def main():
         x = ctx.to_device(np.random.default_rng(F).standard_normal((N, F)) if F > 1 else
                           np.random.default_rng(1).standard_normal(N))
         y = ctx.empty((N, F) if F > 1 else (N,), np.float64)
-        for _ in range(3):
-            eng.spmv(col, vals[2], x, y, uniform_n=n)
-        ctx.synchronize()
-        reps = 50
-        ctx.timer_start()
-        for _ in range(reps):
-            eng.spmv(col, vals[2], x, y, uniform_n=n)
-        t = ctx.timer_stop() / reps
-        bytes_alg = nnz * 12 + 8 * F * (N + N)
-        spmv[f"F{F}"] = {"ms": t, "GB_s": bytes_alg / (t * 1e-3) / 1e9, "bytes": bytes_alg,
-                         "frac_hbm_peak": bytes_alg / (t * 1e-3) / 1e9 / PEAK_HBM_GBS}
-        log(f"  spmv F={F}: {t * 1e3:.1f} us, {spmv[f'F{F}']['GB_s']:.0f} GB/s")
+        for tag, od in (("", None),):
+            for _ in range(3):
+                eng.spmv(col, vals[2], x, y, uniform_n=n, order=od)
+            ctx.synchronize()
+            reps = 50
+            ctx.timer_start()
+            for _ in range(reps):
+                eng.spmv(col, vals[2], x, y, uniform_n=n, order=od)
+            t = ctx.timer_stop() / reps
+            bytes_alg = nnz * 12 + 8 * F * (N + N) + (4 * N if od is not None else 0)
+            spmv[f"F{F}{tag}"] = {"ms": t, "GB_s": bytes_alg / (t * 1e-3) / 1e9, "bytes": bytes_alg,
+                                  "frac_hbm_peak": bytes_alg / (t * 1e-3) / 1e9 / PEAK_HBM_GBS}
+            log(f"  spmv F={F}{tag}: {t * 1e3:.1f} us, {spmv[f'F{F}{tag}']['GB_s']:.0f} GB/s")
     a = ctx.empty((1 << 27,), np.float64)
     b = ctx.empty((1 << 27,), np.float64)
     b.copy_from_device(a)
@@ -238,6 +241,18 @@ def main():
         cpu["spmv"] = cpu_spmv
         log(f"  CPU SpMV {json.dumps(cpu_spmv)}")
 
+    extras = None
+    if not args.no_extras:
+        del col, vals, st_d, Xd
+        extras = {}
+        for name, fn in (("hyperviscosity_4m", extra_hyper4m), ("uneven_batch_64", extra_batch),
+                         ("advdiff_ssprk3_4m", extra_advdiff)):
+            try:
+                extras[name] = fn(rbffd_amd, ctx, eng, args.no_cpu)
+            except Exception as e:  # a failed extra is reported, the headline stands
+                extras[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"  {name}: {json.dumps(extras[name])}")
+
     G = r64["lanes"]
     out = {
         "metric": METRIC, "value": value, "unit": "stencils/s", "n_gpus": 1, "steps": args.steps,
@@ -253,9 +268,148 @@ def main():
         "kernel_only_stencils_per_s": N / (r64["ms_solve"] * 1e-3),
         "ms_knn": r64["ms_knn"], "ms_solve": r64["ms_solve"], "lanes_per_stencil": G,
         "end_to_end": e2e, "accuracy": accuracy,
-        "spmv": spmv, "mixed": mixed, "cpu_baseline": cpu, "device": info,
+        "spmv": spmv, "mixed": mixed, "cpu_baseline": cpu, "extras": extras, "device": info,
     }
     print(json.dumps(out))
+
+
+def extra_hyper4m(rbffd_amd, ctx, eng, no_cpu):
+    """4,194,304 nodes (2048^2, +-0.3h), r^9, degree 6, n = 57: Lap^3, and the
+    reference-semantics pair d^6/dx^6, d^6/dy^6, FP64 LU."""
+    from rbffd_amd.nodes import config_nodes
+
+    log("extra: hyperviscosity, 4M nodes, n = 57")
+    X, prm = config_nodes("hyper4m")
+    N, n = len(X), prm["n"]
+    Xd = ctx.to_device(X)
+    st = ctx.empty((N, n), np.int32)
+    res = {"nodes": N, "stencil": n, "polydeg": prm["polydeg"], "phs_power": prm["p"]}
+    outs = {}
+    for tag, ops, k in (("lap3", ["Lapk"], 3), ("dxk_dyk6", ["Dxk", "Dyk"], 6)):
+        col = ctx.empty((N, n), np.int32)
+        vals = [ctx.empty((N, n), np.float64) for _ in ops]
+        eng.generate(Xd, Xd, prm["p"], n, prm["polydeg"], ops, k=k, colidx=col, vals=vals, stencils_out=st)
+        ctx.synchronize()
+        ctx.timer_start()
+        _, _, stats = eng.generate(Xd, Xd, prm["p"], n, prm["polydeg"], ops, k=k, colidx=col, vals=vals,
+                                   stencils_out=st)
+        ms = ctx.timer_stop()
+        nsys = n + 28
+        flops = (2.0 / 3.0) * nsys**3 + 2.0 * nsys**2 * len(ops)
+        res[tag] = {"ms": ms, "stencils_per_s": N / (ms * 1e-3), "ms_solve": stats["ms_solve"],
+                    "ms_knn": stats["ms_knn"], "lanes_per_stencil": stats["lanes_per_stencil"],
+                    "solve_tflops": N * flops / (stats["ms_solve"] * 1e-3) / 1e12}
+        outs[tag] = (ops, k, col, [v.get() for v in vals])
+        del col, vals
+    if not no_cpu:
+        from oracle import oracle as O
+
+        rows = np.arange(0, N, 16384, dtype=np.int64)
+        stn, _ = O.knn(X, X[rows], n, nthreads=16)
+        full = np.zeros((N, n), np.int32)
+        full[rows] = stn
+        order = np.argsort(stn, axis=1, kind="stable")
+        for tag, (ops, k, _, hv) in outs.items():
+            opl = [(O.OP_LAPK if o == "Lapk" else (O.OP_XK if o == "Dxk" else O.OP_YK), k) for o in ops]
+            Wq, _, _ = O.weights(X, X, full, None, prm["p"], prm["polydeg"], opl, quad=True, rows=rows, nthreads=16)
+            Wl, _, _ = O.weights(X, X, full, None, prm["p"], prm["polydeg"], opl, solve="lu", rows=rows,
+                                 nthreads=16)
+            acc = {}
+            for o, name in enumerate(ops):
+                ref = np.take_along_axis(Wq[:, o, :], order, 1)
+                lu = np.take_along_axis(Wl[:, o, :], order, 1)
+                acc[name] = {"gpu_vs_quad_max": row_errors(hv[o][rows], ref)[0],
+                             "fp64_lu_oracle_vs_quad_max": row_errors(lu, ref)[0]}
+            res[tag]["accuracy_rows"] = len(rows)
+            res[tag]["accuracy"] = acc
+    res["mixed_precision"] = "not compiled for n + P = 85 (the FP32 path covers n + P <= 64)"
+    return res
+
+
+def extra_batch(rbffd_amd, ctx, eng, no_cpu):
+    """64 node sets, 10k-500k nodes each, degree 2-5 (n = 13, 21, 31, 43), ghost
+    nodes with method-2 groups, one captured graph per batch."""
+    from rbffd_amd.nodes import batch_sets
+
+    log("extra: uneven batch of 64 node sets")
+    specs = batch_sets()
+    sets, total = [], 0
+    for S in specs:
+        X = S["X"]
+        xg, qg = rbffd_amd.boundary_groups(len(X), S["X_idx_in"], S["X_idx_bc"], S["X_idx_bc_g"])
+        sets.append(dict(X=ctx.to_device(X), p=S["p"], n=S["n"], polydeg=S["polydeg"], xgroup=ctx.to_device(xg),
+                         qgroup=ctx.to_device(qg)))
+        total += len(X)
+    ops = ["Dx", "Dy", "Lap"]
+    out, st, stats = eng.generate_batch(sets, ops, use_graph=True)  # eager run, tuning, capture
+    for S, (c, vs, s) in zip(sets, out):
+        S["colidx"], S["vals"], S["status"] = c, vs, s
+    eng.generate_batch(sets, ops, use_graph=True)
+    times = []
+    for _ in range(3):
+        _, st, stats = eng.generate_batch(sets, ops, use_graph=True)
+        times.append(stats["ms_solve"])
+    t_graph = float(np.median(times))
+    _, _, s2 = eng.generate_batch(sets, ops, use_graph=False)
+    t_eager = s2["ms_solve"]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for S in sets:  # one generation call per set, as a caller of the single-set API would
+        eng.generate(S["X"], S["X"], S["p"], S["n"], S["polydeg"], ops, colidx=S["colidx"], vals=S["vals"])
+    ctx.synchronize()
+    t_loop = (time.perf_counter() - t0) * 1e3
+    return {"sets": len(sets), "nodes": total, "failed_sets": int((st != 0).sum()),
+            "graph_ms": t_graph, "stencils_per_s_graph": total / (t_graph * 1e-3),
+            "eager_one_stream_ms": t_eager, "per_set_calls_ms": t_loop,
+            "stencil_sizes": sorted({S["n"] for S in specs})}
+
+
+def extra_advdiff(rbffd_amd, ctx, eng, no_cpu):
+    """4,194,304 nodes, n = 31 Dx, Dy, Lap and Lap^3 fused into one operator,
+    8 Gaussian fields, 1000 SSP-RK3 steps with FP64 and with FP32 values."""
+    from rbffd_amd.nodes import config_nodes
+
+    log("extra: advection-diffusion, 4M nodes, 8 fields, 1000 SSP-RK3 steps")
+    X, prm = config_nodes("advdiff4m")
+    N, n = len(X), prm["n"]
+    h = 1.0 / 2047
+    Xd = ctx.to_device(X)
+    col, vals, _ = eng.generate(Xd, Xd, prm["p"], n, prm["polydeg"], ["Dx", "Dy", "Lap", "Lapk"], k=3)
+    gamma, kappa = 1e-8, 1e-3
+    coef = [-1.0, -0.5, kappa, -gamma * h**6]
+    L64 = eng.combine(vals, coef)
+    L32 = ctx.empty((N, n), np.float32)
+    eng.combine(vals, coef, out32=L32)
+    del vals
+    rng = np.random.default_rng(5)
+    cen = rng.uniform(0.2, 0.8, size=(8, 2))
+    u0 = np.ascontiguousarray(np.stack([np.exp(-((X[:, 0] - cx) ** 2 + (X[:, 1] - cy) ** 2) / 0.05**2)
+                                        for cx, cy in cen], 1))
+    edge = (X[:, 0] <= 0) | (X[:, 0] >= 1) | (X[:, 1] <= 0) | (X[:, 1] >= 1)
+    fixed = ctx.to_device(edge.astype(np.uint8))
+    dt, steps = 0.05 * h, 1000
+    work = ctx.empty((2 * N * 8,), np.float64)
+    res = {"nodes": N, "fields": 8, "steps": steps, "dt": dt, "kappa": kappa, "gamma_h6": gamma * h**6,
+           "velocity": [1.0, 0.5], "bc": "boundary nodes held at their initial values",
+           "dt_note": "dt = 0.05h: kappa * dt * rho(Lap) ~ 2.05 (rho ~ 20/h^2) keeps SSP-RK3 stable; 0.2h gives ~8"}
+    finals = {}
+    nnz = N * n
+    order = None  # the grid's row-major node order already keeps stencils local
+    for tag, Lv, vb in (("fp64", L64, 8), ("fp32_values", L32, 4)):
+        u = ctx.to_device(u0)
+        eng.ssprk3(col, Lv, u, dt, 2, uniform_n=n, fixed=fixed, work=work, order=order)  # warm-up
+        u.copy_from_host(u0)
+        ctx.synchronize()
+        ctx.timer_start()
+        eng.ssprk3(col, Lv, u, dt, steps, uniform_n=n, fixed=fixed, work=work, order=order)
+        ms = ctx.timer_stop()
+        stage_bytes = nnz * (4 + vb) + 3 * 8 * 8 * N
+        res[tag] = {"ms": ms, "ms_per_step": ms / steps, "stage_GB_s": 3 * steps * stage_bytes / (ms * 1e-3) / 1e9}
+        finals[tag] = u.get()
+    ref = finals["fp64"]
+    res["fp32_values_final_rel_err"] = float(np.abs(finals["fp32_values"] - ref).max() / np.abs(ref).max())
+    res["final_max_abs"] = float(np.abs(ref).max())
+    return res
 
 
 if __name__ == "__main__":

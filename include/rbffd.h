@@ -178,12 +178,43 @@ int rbffd_csr_to_csc_device(rbffd_context *ctx, int64_t M, int32_t n, int64_t nc
  * y = alpha A x + beta y for F fields (1, 2, 4, 8) stored node-major
  * (x[col F + f]); A is CSR with int32 columns and FP64 or FP32 values
  * (accumulated in FP64).  uniform_n > 0: every row has uniform_n entries and
- * rowptr may be NULL.  Replaces SparseMatrixCSC * Vector in the reference's
+ * rowptr may be NULL.  row_order (may be NULL): the rows run in this order
+ * (a permutation of 0..M-1, e.g. from rbffd_locality_order_device).  Replaces SparseMatrixCSC * Vector in the reference's
  * harnesses (test/poisson_test.jl:121, examples/adv_diff_test.jl:151-152).
  * Asynchronous. */
 int rbffd_spmv_device(rbffd_context *ctx, int64_t M, const int64_t *rowptr, const int32_t *col,
                       const void *vals, int32_t vals_type, int32_t F, const double *x, double *y,
-                      double alpha, double beta, int32_t uniform_n);
+                      double alpha, double beta, int32_t uniform_n, const int32_t *row_order);
+
+/* A locality order of N nodes (Morton order of their coordinates): with it as
+ * row_order, the rows of one workgroup are a compact patch of the domain and
+ * share most of the x values they gather.  Results are identical with or
+ * without an order.  No reference counterpart.  Asynchronous. */
+int rbffd_locality_order_device(rbffd_context *ctx, const double *X, int64_t N, int32_t *order);
+
+/* ---------------------------------------------------- operator algebra
+ * out = sum_o coef[o] vals[o] over nnz entries of operators that share one
+ * pattern (the operators of one generation call do), e.g. a fused
+ * advection-diffusion operator from Dx, Dy, Lap and Lap^k.  vals: host array
+ * of nops device pointers; coef: host array.  out (FP64) and out32 (its FP32
+ * rounding) may each be NULL.  Replaces the SparseMatrixCSC sums of the
+ * harness (examples/adv_diff_test.jl:101,151-152).  Asynchronous. */
+int rbffd_combine_device(rbffd_context *ctx, int64_t nnz, int32_t nops, const double *const *vals,
+                         const double *coef, double *out, float *out32);
+
+/* ----------------------------------------------------- time stepping
+ * `steps` steps of the three-stage strong-stability-preserving Runge-Kutta
+ * method (Shu-Osher form) for du/dt = L u on F fields stored node-major, each
+ * stage one fused pass y = a u + b w + c (L w) over the CSR operator L (N x N,
+ * FP64 or FP32 values, FP64 accumulation).  Rows with fixed[i] != 0 keep their
+ * value (Dirichlet rows; fixed may be NULL).  row_order as in
+ * rbffd_spmv_device.  u (N F) is advanced in place;
+ * work holds 2 N F doubles.  Replaces the explicit time integration of the
+ * harness (examples/adv_diff_test.jl:190-198, an adaptive SSPRK43 there).
+ * Asynchronous. */
+int rbffd_ssprk3_device(rbffd_context *ctx, int64_t N, const int64_t *rowptr, const int32_t *col, const void *vals,
+                        int32_t vals_type, int32_t uniform_n, const int32_t *row_order, int32_t F, double *u,
+                        double *work, const uint8_t *fixed, double dt, int64_t steps);
 
 /* ------------------------------------------------------ uneven batch
  * Many independent node sets, each with its own stencil size, polynomial

@@ -34,16 +34,24 @@ struct rbffd_arena {
 
 constexpr int kLanes = 4;  // parallel branches of a captured batch
 
+// Workspace slots of the context (one grow-only device buffer each).
+enum Slot {
+    S_BBOX_PART, S_BBOX, S_FLAG, S_XGROUP, S_QGROUP, S_STENCILS, S_Y2X, S_STATUS, S_ROWS, S_CSC_COUNTS,
+    S_CSC_PTR, S_CSC_FILL, S_CSC_PERM, S_CSC_ROW, S_H_X, S_H_Y, S_H_COL, S_H_VALS, S_H_CVALS,
+    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS, S_MORTON_KEY, S_MORTON_IDX, S_NSLOT
+};
+
 struct rbffd_context {
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     using Buf = rbffd_buf;
-    Buf bufs[32];
+    Buf bufs[S_NSLOT];
     hipEvent_t ev[8] = {};
     hipEvent_t tune_ev[2] = {};
     // autotuner: (n, P, right-hand sides) -> lanes per stencil of the FP64 kernel
     std::map<std::tuple<int, int, int>, int> lu_group;
+    int last_group = 0;  // lanes per stencil of the last solve launch
     // neighbour-search workspace of the single-set calls, and one per batch branch
     rbffd_arena knn_arena;
     rbffd_arena lane_arena[kLanes];
@@ -59,12 +67,6 @@ namespace {
 
 thread_local std::string g_last_error;
 
-enum Slot {
-    S_BBOX_PART, S_BBOX, S_COUNTS, S_START, S_CELL_OF, S_FILL, S_CELL_XY, S_CELL_IDX, S_CELL_GRP,
-    S_SCAN_TMP, S_FLAG, S_XGROUP, S_QGROUP, S_STENCILS, S_Y2X, S_STATUS, S_ROWS, S_CSC_COUNTS,
-    S_CSC_PTR, S_CSC_FILL, S_CSC_PERM, S_CSC_ROW, S_H_X, S_H_Y, S_H_COL, S_H_VALS, S_H_CVALS,
-    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS
-};
 
 template <class F>
 int guarded(F &&f)
@@ -101,7 +103,7 @@ void *grow(rbffd_buf &b, size_t bytes, bool may_alloc = true)
     return b.p;
 }
 
-void *ws(rbffd_context *c, int slot, size_t bytes) { return grow(c->bufs[slot], bytes); }
+void *ws(rbffd_context *c, Slot slot, size_t bytes) { return grow(c->bufs[slot], bytes); }
 
 void check_ctx(rbffd_context *c)
 {
@@ -311,6 +313,7 @@ void solves_enqueue(rbffd_context *c, const double *X, const double *Y, long M, 
         a.max_iter = 8;
         a.tol = 4e-16;
         a.group = precision == RBFFD_PREC_FP64 ? choose_group(c, a, may_tune) : 0;
+        if (o0 == 0) c->last_group = precision == RBFFD_PREC_FP64 ? a.group : 64;
         rbffd::launch_solve(a, precision, s);
         RBFFD_HIP(hipGetLastError());
     }
@@ -434,11 +437,7 @@ void generate_impl(rbffd_context *c, const double *X, long N, const double *Y, l
         stats->ms_knn = t01;
         stats->ms_solve = t12;
         stats->ms_fallback = t23;
-        int groups[rbffd::kMaxGroups];
-        rbffd::solve_groups(n, n_monomials(polydeg), 0, groups);
-        const int rb = std::min(nops, 3) < nops ? rbffd::kMaxRhs : nops;
-        auto it = c->lu_group.find(std::make_tuple(n, n_monomials(polydeg), rb));
-        stats->lanes_per_stencil = it != c->lu_group.end() ? it->second : groups[0];
+        stats->lanes_per_stencil = c->last_group;
     }
     if (hc[1] > 0)
         throw Error{RBFFD_ERR_SINGULAR, std::to_string(hc[1]) + " stencil systems are exactly singular"};
@@ -854,7 +853,7 @@ int rbffd_csr_to_csc_device(rbffd_context *c, int64_t M, int32_t n, int64_t ncol
 
 int rbffd_spmv_device(rbffd_context *c, int64_t M, const int64_t *rowptr, const int32_t *col, const void *vals,
                       int32_t vals_type, int32_t F, const double *x, double *y, double alpha, double beta,
-                      int32_t uniform_n)
+                      int32_t uniform_n, const int32_t *row_order)
 {
     return guarded([&] {
         check_ctx(c);
@@ -862,7 +861,7 @@ int rbffd_spmv_device(rbffd_context *c, int64_t M, const int64_t *rowptr, const 
         if (uniform_n <= 0 && !rowptr) throw Error{RBFFD_ERR_ARG, "rowptr needed for non-uniform rows"};
         if (vals_type != RBFFD_F64 && vals_type != RBFFD_F32) throw Error{RBFFD_ERR_ARG, "unknown value type"};
         rbffd::launch_spmv(M, rowptr, col, vals, vals_type == RBFFD_F32, F, x, y, alpha, beta, uniform_n,
-                           c->stream);
+                           row_order, c->stream);
         RBFFD_HIP(hipGetLastError());
     });
 }
@@ -929,6 +928,70 @@ int rbffd_generate_batch_device(rbffd_context *c, int32_t nsets, const rbffd_bat
                                 rbffd_stats *stats)
 {
     return guarded([&] { batch_impl(c, nsets, sets, nops, ops, opk, use_graph, set_status, stats); });
+}
+
+int rbffd_combine_device(rbffd_context *c, int64_t nnz, int32_t nops, const double *const *vals,
+                         const double *coef, double *out, float *out32)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (nnz < 0 || nops < 1 || nops > rbffd::kMaxOps || !vals || !coef || (!out && !out32))
+            throw Error{RBFFD_ERR_ARG, "combine: bad arguments"};
+        rbffd::CombineTerms t{};
+        t.nops = nops;
+        for (int o = 0; o < nops; ++o) {
+            if (!vals[o]) throw Error{RBFFD_ERR_ARG, "combine: null value array"};
+            t.vals[o] = vals[o];
+            t.coef[o] = coef[o];
+        }
+        rbffd::launch_combine(nnz, t, out, out32, c->stream);
+        RBFFD_HIP(hipGetLastError());
+    });
+}
+
+int rbffd_ssprk3_device(rbffd_context *c, int64_t N, const int64_t *rowptr, const int32_t *col, const void *vals,
+                        int32_t vals_type, int32_t uniform_n, const int32_t *row_order, int32_t F, double *u,
+                        double *work, const uint8_t *fixed, double dt, int64_t steps)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (N < 0 || !col || !vals || !u || !work || steps < 0) throw Error{RBFFD_ERR_ARG, "ssprk3: bad arguments"};
+        if (uniform_n <= 0 && !rowptr) throw Error{RBFFD_ERR_ARG, "ssprk3: rowptr or uniform_n needed"};
+        if (vals_type != RBFFD_F64 && vals_type != RBFFD_F32) throw Error{RBFFD_ERR_ARG, "ssprk3: value type"};
+        if (F != 1 && F != 2 && F != 4 && F != 8) throw Error{RBFFD_ERR_UNSUPPORTED, "ssprk3: fields must be 1, 2, 4 or 8"};
+        hipStream_t s = c->stream;
+        const int f32 = vals_type == RBFFD_F32;
+        double *u1 = work, *u2 = work + (size_t)N * F;
+        for (int64_t k = 0; k < steps; ++k) {
+            // Shu-Osher form of SSP-RK3 (Gottlieb, Shu and Tadmor 2001)
+            const int *o = row_order;
+            rbffd::launch_rk_stage(N, rowptr, col, vals, f32, F, uniform_n, o, u, u, u1, fixed, 0.0, 1.0, dt, s);
+            rbffd::launch_rk_stage(N, rowptr, col, vals, f32, F, uniform_n, o, u1, u, u2, fixed, 0.75, 0.25,
+                                   0.25 * dt, s);
+            rbffd::launch_rk_stage(N, rowptr, col, vals, f32, F, uniform_n, o, u2, u, u, fixed, 1.0 / 3.0,
+                                   2.0 / 3.0, 2.0 / 3.0 * dt, s);
+        }
+        RBFFD_HIP(hipGetLastError());
+    });
+}
+
+int rbffd_locality_order_device(rbffd_context *c, const double *X, int64_t N, int32_t *order)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!X || !order || N < 1) throw Error{RBFFD_ERR_ARG, "locality_order: bad arguments"};
+        hipStream_t s = c->stream;
+        const int nb = (int)std::min<long>(1024, (N + 255) / 256);
+        double *part = (double *)ws(c, S_BBOX_PART, sizeof(double) * 5 * nb);
+        double *box = (double *)ws(c, S_BBOX, sizeof(double) * 5);
+        rbffd::launch_bbox(X, N, box, part, nb, s);
+        auto *key = (uint32_t *)ws(c, S_MORTON_KEY, sizeof(uint32_t) * 2 * N);
+        int *idx = (int *)ws(c, S_MORTON_IDX, sizeof(int) * N);
+        const size_t tb = rbffd::morton_tmp_bytes(N);
+        void *tmp = ws(c, S_SCAN_TMP2, tb);
+        rbffd::launch_morton_order(X, N, box, key, key + N, idx, order, tmp, tb, s);
+        RBFFD_HIP(hipGetLastError());
+    });
 }
 
 }  // extern "C"

@@ -107,8 +107,28 @@ void launch_i32_to_i64(const int *src, long n, int64_t *dst, int64_t add, hipStr
 
 // y[M x F] = A x[N x F], CSR with int32 columns.  vals_f32 selects FP32 values
 // (accumulated in FP64).
+// Rows run in `order` (a permutation of 0..M-1) when it is not NULL.
 void launch_spmv(long M, const int64_t *rowptr, const int *col, const void *vals, int vals_f32, int F,
-                 const double *x, double *y, double alpha, double beta, int uniform_n, hipStream_t s);
+                 const double *x, double *y, double alpha, double beta, int uniform_n, const int *order,
+                 hipStream_t s);
 void launch_f64_to_f32(const double *src, long n, float *dst, hipStream_t s);
+
+// One SSP-RK stage out = a u + b w + c (L w) (rows with fixed != 0 keep u).
+void launch_rk_stage(long M, const int64_t *rowptr, const int *col, const void *vals, int vals_f32, int F,
+                     int uniform_n, const int *order, const double *w, const double *u, double *out,
+                     const uint8_t *fixed, double a, double b, double c, hipStream_t s);
+// Rows of N nodes in Morton order of their coordinates (box from launch_bbox);
+// key, key_sorted, idx: N-entry workspaces; tmp of morton_tmp_bytes(N).
+size_t morton_tmp_bytes(long N);
+void launch_morton_order(const double *X, long N, const double *box, uint32_t *key, uint32_t *key_sorted,
+                         int *idx, int *order, void *tmp, size_t tmp_bytes, hipStream_t s);
+// Terms of a linear combination of value arrays sharing one pattern.
+struct CombineTerms {
+    int nops;
+    const double *vals[kMaxOps];
+    double coef[kMaxOps];
+};
+// out = sum_o coef[o] vals[o] (FP64, may be NULL) and its FP32 rounding (may be NULL).
+void launch_combine(long nnz, const CombineTerms &t, double *out, float *out32, hipStream_t s);
 
 }  // namespace rbffd

@@ -14,6 +14,8 @@
 // (x[col*F + f]) so one gather brings all F fields of a node.
 #include <cstdlib>
 
+#include <hipcub/hipcub.hpp>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -153,14 +155,16 @@ __device__ __forceinline__ double group_sum_f64(double v)
 template <typename TV, int G, int F, bool UNIFORM>
 __global__ __launch_bounds__(256) void spmv_kernel(long M, const int64_t *__restrict__ rowptr,
                                                    const int *__restrict__ col, const TV *__restrict__ val,
-                                                   int n_uniform, const double *__restrict__ x,
-                                                   double *__restrict__ y, double alpha, double beta)
+                                                   int n_uniform, const int *__restrict__ order,
+                                                   const double *__restrict__ x, double *__restrict__ y, double alpha,
+                                                   double beta)
 {
     const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const long row = gid / G;
     const int sub = (int)(gid % G);
     // rows past the end still take part in the shuffles below
-    const bool live = row < M;
+    const bool live = gid / G < M;
+    // rows run in `order` (a locality order) when one is given
+    const long row = !live ? M : (order ? (long)order[gid / G] : gid / G);
     long p0 = 0, p1 = 0;
     if (live) {
         if (UNIFORM) { p0 = row * (long)n_uniform; p1 = p0 + n_uniform; }
@@ -191,15 +195,15 @@ __global__ __launch_bounds__(256) void spmv_kernel(long M, const int64_t *__rest
 }
 
 template <typename TV, int G, int F>
-static void spmv_launch_t(long M, const int64_t *rowptr, const int *col, const TV *val, int nu,
+static void spmv_launch_t(long M, const int64_t *rowptr, const int *col, const TV *val, int nu, const int *order,
                           const double *x, double *y, double alpha, double beta, hipStream_t s)
 {
     const long threads = M * G;
     const unsigned blocks = (unsigned)((threads + 255) / 256);
     if (nu > 0)
-        spmv_kernel<TV, G, F, true><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, x, y, alpha, beta);
+        spmv_kernel<TV, G, F, true><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, x, y, alpha, beta);
     else
-        spmv_kernel<TV, G, F, false><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, x, y, alpha, beta);
+        spmv_kernel<TV, G, F, false><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, x, y, alpha, beta);
 }
 
 // Lanes per row: 16 (one field or several), or RBFFD_SPMV_LANES /
@@ -213,39 +217,183 @@ static int lanes_env(const char *name, int dflt)
 
 template <typename TV, int F>
 static void spmv_lanes_t(int G, long M, const int64_t *rowptr, const int *col, const TV *val, int nu,
-                         const double *x, double *y, double alpha, double beta, hipStream_t s)
+                         const int *order, const double *x, double *y, double alpha, double beta, hipStream_t s)
 {
     switch (G) {
-    case 4: spmv_launch_t<TV, 4, F>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    case 8: spmv_launch_t<TV, 8, F>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    case 32: spmv_launch_t<TV, 32, F>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    default: spmv_launch_t<TV, 16, F>(M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    case 4: spmv_launch_t<TV, 4, F>(M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
+    case 8: spmv_launch_t<TV, 8, F>(M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
+    case 32: spmv_launch_t<TV, 32, F>(M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
+    default: spmv_launch_t<TV, 16, F>(M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
     }
 }
 
 template <typename TV>
 static void spmv_dispatch(long M, const int64_t *rowptr, const int *col, const TV *val, int F, int nu,
-                          const double *x, double *y, double alpha, double beta, hipStream_t s)
+                          const int *order, const double *x, double *y, double alpha, double beta, hipStream_t s)
 {
     static const int g1 = lanes_env("RBFFD_SPMV_LANES", 16);
     static const int gf = lanes_env("RBFFD_SPMM_LANES", 16);
     switch (F) {
-    case 1: spmv_lanes_t<TV, 1>(g1, M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    case 2: spmv_lanes_t<TV, 2>(gf, M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    case 4: spmv_lanes_t<TV, 4>(gf, M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
-    case 8: spmv_lanes_t<TV, 8>(gf, M, rowptr, col, val, nu, x, y, alpha, beta, s); break;
+    case 1: spmv_lanes_t<TV, 1>(g1, M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
+    case 2: spmv_lanes_t<TV, 2>(gf, M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
+    case 4: spmv_lanes_t<TV, 4>(gf, M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
+    case 8: spmv_lanes_t<TV, 8>(gf, M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
     default: throw Error{2, "spmv: fields must be 1, 2, 4 or 8"};
     }
 }
 
 void launch_spmv(long M, const int64_t *rowptr, const int *col, const void *vals, int vals_f32, int F,
-                 const double *x, double *y, double alpha, double beta, int uniform_n, hipStream_t s)
+                 const double *x, double *y, double alpha, double beta, int uniform_n, const int *order,
+                 hipStream_t s)
 {
     if (M <= 0) return;
     if (vals_f32)
-        spmv_dispatch<float>(M, rowptr, col, (const float *)vals, F, uniform_n, x, y, alpha, beta, s);
+        spmv_dispatch<float>(M, rowptr, col, (const float *)vals, F, uniform_n, order, x, y, alpha, beta, s);
     else
-        spmv_dispatch<double>(M, rowptr, col, (const double *)vals, F, uniform_n, x, y, alpha, beta, s);
+        spmv_dispatch<double>(M, rowptr, col, (const double *)vals, F, uniform_n, order, x, y, alpha, beta, s);
+}
+
+// ---- locality order: rows sorted by the Morton code of their node
+// (coordinates quantised to 16 bits over the bounding box)
+__device__ __forceinline__ uint32_t spread_bits(uint32_t v)
+{
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__global__ void morton_kernel(const double *X, long N, const double *box, uint32_t *key, int *idx)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double w = box[2] - box[0], h = box[3] - box[1];
+    const double fx = w > 0 ? (X[2 * i] - box[0]) / w : 0.0;
+    const double fy = h > 0 ? (X[2 * i + 1] - box[1]) / h : 0.0;
+    const uint32_t qx = (uint32_t)fmin(fmax(fx * 65535.0, 0.0), 65535.0);
+    const uint32_t qy = (uint32_t)fmin(fmax(fy * 65535.0, 0.0), 65535.0);
+    key[i] = spread_bits(qx) | (spread_bits(qy) << 1);
+    idx[i] = (int)i;
+}
+
+size_t morton_tmp_bytes(long N)
+{
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                             (const int *)nullptr, (int *)nullptr, (int)N, 0, 32);
+    return bytes;
+}
+
+void launch_morton_order(const double *X, long N, const double *box, uint32_t *key, uint32_t *key_sorted,
+                         int *idx, int *order, void *tmp, size_t tmp_bytes, hipStream_t s)
+{
+    if (N <= 0) return;
+    morton_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, N, box, key, idx);
+    RBFFD_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key_sorted, idx, order, (int)N, 0, 32, s));
+}
+
+// ------------------------------------------------------------ time stepping
+// One Shu-Osher stage: out = a u + b w + c (L w) for F fields, L in CSR with
+// G lanes per row (as spmv_kernel).  Rows with fixed[r] != 0 keep u.  out may
+// be u (each value is read and written by the same lane); out must not be w.
+template <typename TV, int G, int F, bool UNIFORM>
+__global__ __launch_bounds__(256) void rk_stage_kernel(long M, const int64_t *__restrict__ rowptr,
+                                                       const int *__restrict__ col, const TV *__restrict__ val,
+                                                       int n_uniform, const int *__restrict__ order,
+                                                       const double *__restrict__ w, const double *u, double *out,
+                                                       const uint8_t *__restrict__ fixed, double a, double b, double c)
+{
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int sub = (int)(gid % G);
+    const bool live = gid / G < M;
+    const long row = !live ? M : (order ? (long)order[gid / G] : gid / G);
+    long p0 = 0, p1 = 0;
+    if (live) {
+        if (UNIFORM) { p0 = row * (long)n_uniform; p1 = p0 + n_uniform; }
+        else { p0 = rowptr[row]; p1 = rowptr[row + 1]; }
+    }
+    double acc[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc[f] = 0.0;
+    for (long p = p0 + sub; p < p1; p += G) {
+        const double v = (double)val[p];
+        const long cidx = (long)col[p] * F;
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc[f] = __builtin_fma(v, w[cidx + f], acc[f]);
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc[f] = group_sum_f64<G>(acc[f]);
+    if (live) {
+        const bool hold = fixed && fixed[row];
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            if (f % G == sub) {
+                const long i = row * F + f;
+                const double ui = u[i];
+                out[i] = hold ? ui : a * ui + b * w[i] + c * acc[f];
+            }
+        }
+    }
+}
+
+template <typename TV, int F>
+static void rk_stage_t(long M, const int64_t *rowptr, const int *col, const TV *val, int nu, const int *order,
+                       const double *w, const double *u, double *out, const uint8_t *fixed, double a, double b,
+                       double c, hipStream_t s)
+{
+    constexpr int G = 16;
+    const unsigned blocks = (unsigned)((M * G + 255) / 256);
+    if (nu > 0)
+        rk_stage_kernel<TV, G, F, true><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, w, u, out, fixed, a,
+                                                               b, c);
+    else
+        rk_stage_kernel<TV, G, F, false><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, w, u, out, fixed, a,
+                                                                b, c);
+}
+
+template <typename TV>
+static void rk_stage_f(int F, long M, const int64_t *rowptr, const int *col, const TV *val, int nu,
+                       const int *order, const double *w, const double *u, double *out, const uint8_t *fixed,
+                       double a, double b, double c, hipStream_t s)
+{
+    switch (F) {
+    case 1: rk_stage_t<TV, 1>(M, rowptr, col, val, nu, order, w, u, out, fixed, a, b, c, s); break;
+    case 2: rk_stage_t<TV, 2>(M, rowptr, col, val, nu, order, w, u, out, fixed, a, b, c, s); break;
+    case 4: rk_stage_t<TV, 4>(M, rowptr, col, val, nu, order, w, u, out, fixed, a, b, c, s); break;
+    case 8: rk_stage_t<TV, 8>(M, rowptr, col, val, nu, order, w, u, out, fixed, a, b, c, s); break;
+    default: throw Error{2, "time stepping: fields must be 1, 2, 4 or 8"};
+    }
+}
+
+void launch_rk_stage(long M, const int64_t *rowptr, const int *col, const void *vals, int vals_f32, int F,
+                     int uniform_n, const int *order, const double *w, const double *u, double *out,
+                     const uint8_t *fixed, double a, double b, double c, hipStream_t s)
+{
+    if (M <= 0) return;
+    if (vals_f32)
+        rk_stage_f<float>(F, M, rowptr, col, (const float *)vals, uniform_n, order, w, u, out, fixed, a, b, c, s);
+    else
+        rk_stage_f<double>(F, M, rowptr, col, (const double *)vals, uniform_n, order, w, u, out, fixed, a, b, c,
+                           s);
+}
+
+// out = sum_o coef[o] vals[o] (FP64), and its FP32 rounding when out32 is set.
+__global__ void combine_kernel(long nnz, CombineTerms t, double *out, float *out32)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    double r = 0.0;
+    for (int o = 0; o < t.nops; ++o) r = __builtin_fma(t.coef[o], t.vals[o][i], r);
+    if (out) out[i] = r;
+    if (out32) out32[i] = (float)r;
+}
+
+void launch_combine(long nnz, const CombineTerms &t, double *out, float *out32, hipStream_t s)
+{
+    if (nnz <= 0) return;
+    combine_kernel<<<(unsigned)((nnz + 255) / 256), 256, 0, s>>>(nnz, t, out, out32);
 }
 
 }  // namespace rbffd

@@ -174,7 +174,9 @@ struct LuShape {
     static constexpr int UTOT = NS * WIDTH - NS * (NS - 1) / 2;
     // during assembly: P (node-major), then Phi as a packed upper triangle
     static constexpr int PHI0 = NN * NP;
-    static constexpr int ASM = PHI0 + NN * (NN + 1) / 2;
+    // scaled node coordinates (x, y) pairs after Phi, until elimination starts
+    static constexpr int XI0 = PHI0 + NN * (NN + 1) / 2;
+    static constexpr int ASM = XI0 + 2 * NN;
     // Phi(a, b), a <= b, at PHI0 + tri(a) + b
     static constexpr __host__ __device__ int tri(int a) { return a * (2 * NN - a + 1) / 2 - a; }
     // +kStepBlock: a step writes up to kStepBlock-1 stale values past its row;
@@ -187,8 +189,7 @@ struct LuShape {
 template <int NN, int NP, int R, int G>
 struct alignas(16) LuLDS {
     using T = LuShape<NN, NP, R, G>;
-    double xi[T::S][NN][2];
-    double u[T::S][T::USTRIDE];  // P and Phi during assembly, then the U factor
+    double u[T::S][T::USTRIDE];  // P, Phi and xi during assembly, then the U factor
     double rp[T::S];              // reciprocal of the current pivot
     double x[T::S][R];            // the unknown solved last, per right-hand side
     int g[T::S][NN];
@@ -264,8 +265,8 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             dxn[r] = xi_x;
             dyn[r] = xi_y;
             if (i < NN) {
-                L.xi[grp][i][0] = xi_x;
-                L.xi[grp][i][1] = xi_y;
+                L.u[grp][T::XI0 + 2 * i] = xi_x;
+                L.u[grp][T::XI0 + 2 * i + 1] = xi_y;
                 L.g[grp][i] = gid[r];
             }
         }
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
                 for (int d = 1; d <= NN / 2; ++d) {
                     int j = i + d;
                     if (j >= NN) j -= NN;
-                    const double ddx = dxn[r] - L.xi[grp][j][0], ddy = dyn[r] - L.xi[grp][j][1];
+                    const double ddx = dxn[r] - L.u[grp][T::XI0 + 2 * j], ddy = dyn[r] - L.u[grp][T::XI0 + 2 * j + 1];
                     const int lo = i < j ? i : j, hi = i < j ? j : i;
                     phi[T::tri(lo) + hi] = phs_r2(m, sq_sum(ddx, ddy));
                 }
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) rhs[rr][cc] = 0.0;
         if (i < NN) {
-            const double xix = L.xi[grp][i][0], xiy = L.xi[grp][i][1];
+            const double xix = L.u[grp][T::XI0 + 2 * i], xiy = L.u[grp][T::XI0 + 2 * i + 1];
             double ux, uy;
             {
 #pragma clang fp contract(off)

@@ -115,7 +115,10 @@ SIGNATURES = {
                                              ctypes.POINTER(rbffd_stats)]),
     "rbffd_csr_to_csc_device": (ctypes.c_int, [_V, _I64, _I32, _I64, _V, _I32, _V, _V, _V, _V, _I32,
                                                ctypes.POINTER(_I64)]),
-    "rbffd_spmv_device": (ctypes.c_int, [_V, _I64, _V, _V, _V, _I32, _I32, _V, _V, _D, _D, _I32]),
+    "rbffd_spmv_device": (ctypes.c_int, [_V, _I64, _V, _V, _V, _I32, _I32, _V, _V, _D, _D, _I32, _V]),
+    "rbffd_locality_order_device": (ctypes.c_int, [_V, _V, _I64, _V]),
+    "rbffd_combine_device": (ctypes.c_int, [_V, _I64, _I32, _V, _V, _V, _V]),
+    "rbffd_ssprk3_device": (ctypes.c_int, [_V, _I64, _V, _V, _V, _I32, _I32, _V, _I32, _V, _V, _V, _D, _I64]),
     "rbffd_boundary_groups": (ctypes.c_int, [ctypes.POINTER(rbffd_boundary), _I64, _V, _V]),
     "rbffd_generate_batch_device": (ctypes.c_int, [_V, _I32, ctypes.POINTER(rbffd_batch_set), _I32, _V, _V, _I32,
                                                    _V, ctypes.POINTER(rbffd_stats)]),

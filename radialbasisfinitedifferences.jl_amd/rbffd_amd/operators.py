@@ -259,6 +259,32 @@ class Engine:
         check(code)
         return out, set_status, stats.as_dict()
 
+    def combine(self, vals, coef, out=None, out32=None):
+        """out = sum_o coef[o] vals[o] for value arrays sharing one pattern;
+        out32 receives the FP32 rounding."""
+        nnz = vals[0].size
+        if out is None and out32 is None:
+            out = self.ctx.empty(vals[0].shape, np.float64)
+        vp = (ctypes.c_void_p * len(vals))(*[v.ptr for v in vals])
+        cf = np.ascontiguousarray(coef, np.float64)
+        check(self.L.rbffd_combine_device(self.ctx.handle, nnz, len(vals), ctypes.cast(vp, ctypes.c_void_p),
+                                          _vp(cf), out.ptr if out is not None else None,
+                                          out32.ptr if out32 is not None else None))
+        return out if out is not None else out32
+
+    def ssprk3(self, col, vals, u, dt, steps, uniform_n=0, rowptr=None, fixed=None, work=None, order=None):
+        """Advance u (N or N x F, device) by `steps` SSP-RK3 steps of du/dt = L u."""
+        N = u.shape[0]
+        F = 1 if len(u.shape) == 1 else u.shape[1]
+        if work is None:
+            work = self.ctx.empty((2 * N * F,), np.float64)
+        vt = 1 if vals.dtype == np.float32 else 0
+        check(self.L.rbffd_ssprk3_device(self.ctx.handle, N, rowptr.ptr if rowptr is not None else None, col.ptr,
+                                         vals.ptr, vt, int(uniform_n), order.ptr if order is not None else None, F,
+                                         u.ptr, work.ptr,
+                                         fixed.ptr if fixed is not None else None, float(dt), int(steps)))
+        return u
+
     def csr_to_csc(self, colidx, vals, ncols, index_base=0):
         M, n = colidx.shape
         colptr = self.ctx.empty((ncols + 1,), np.int64)
@@ -272,10 +298,17 @@ class Engine:
                                              ctypes.cast(cp, ctypes.c_void_p), int(index_base), ctypes.byref(used)))
         return colptr, rowval, cvals, int(used.value)
 
-    def spmv(self, col, vals, x, y, uniform_n=0, rowptr=None, alpha=1.0, beta=0.0):
+    def spmv(self, col, vals, x, y, uniform_n=0, rowptr=None, alpha=1.0, beta=0.0, order=None):
         M = y.shape[0]
         F = 1 if len(x.shape) == 1 else x.shape[1]
         vt = 1 if vals.dtype == np.float32 else 0
         check(self.L.rbffd_spmv_device(self.ctx.handle, M, rowptr.ptr if rowptr is not None else None, col.ptr,
-                                       vals.ptr, vt, F, x.ptr, y.ptr, float(alpha), float(beta), int(uniform_n)))
+                                       vals.ptr, vt, F, x.ptr, y.ptr, float(alpha), float(beta), int(uniform_n),
+                                       order.ptr if order is not None else None))
         return y
+
+    def locality_order(self, X):
+        """Morton order of the nodes of X (device, N x 2), as an int32 device array."""
+        order = self.ctx.empty((X.shape[0],), np.int32)
+        check(self.L.rbffd_locality_order_device(self.ctx.handle, X.ptr, X.shape[0], order.ptr))
+        return order

@@ -44,6 +44,13 @@ def test_spmv_matches_oracle(ctx, O, F, vtype):
     y2 = ctx.to_device(np.ones_like(x))
     eng.spmv(col, vd, xd, y2, rowptr=rpd, alpha=2.0, beta=-1.0)
     assert np.abs(y2.get() - (2 * ref - 1)).max() <= 2e-13 * scale
+    # a locality order changes which rows run together, not the results
+    order = eng.locality_order(Xd)
+    oh = order.get()
+    assert np.array_equal(np.sort(oh), np.arange(N))
+    y3 = ctx.empty(x.shape, np.float64)
+    eng.spmv(col, vd, xd, y3, uniform_n=n, order=order)
+    assert np.array_equal(y3.get(), y)
 
 
 def test_csr_to_csc_matches_sparse(ctx, O):
