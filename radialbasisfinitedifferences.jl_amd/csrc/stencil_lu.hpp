@@ -31,8 +31,11 @@ namespace rbffd {
 template <int A_, int B_>
 struct cmax_ { static constexpr int v = A_ > B_ ? A_ : B_; };
 
-// elimination steps per block of equal static row width
-constexpr int kStepBlock = 8;
+// elimination steps per block of equal static row width: shorter blocks waste
+// fewer columns per step and cost code size (the 16-lane layout, with three
+// rows per lane, keeps 4 to stay inside the instruction cache)
+template <int G>
+constexpr int step_block() { return G == 32 ? 2 : 4; }
 
 // a * b + c into a register distinct from c's.  The compiler's own lowering
 // of the sliding update (row[j] = fma(., ., row[j+1]) in a loop) is a copy
@@ -179,10 +182,11 @@ struct LuShape {
     static constexpr int ASM = XI0 + 2 * NN;
     // Phi(a, b), a <= b, at PHI0 + tri(a) + b
     static constexpr __host__ __device__ int tri(int a) { return a * (2 * NN - a + 1) / 2 - a; }
-    // +kStepBlock: a step writes up to kStepBlock-1 stale values past its row;
+    static constexpr int SB = step_block<G>();
+    // +SB: a step writes up to SB-1 stale values past its row;
     // +2 keeps the groups' slots off one LDS bank
-    static constexpr int USTRIDE = cmax_<UTOT, ASM>::v + kStepBlock + 2;
-    static constexpr int NBLK = (NS + kStepBlock - 1) / kStepBlock;
+    static constexpr int USTRIDE = cmax_<UTOT, ASM>::v + SB + 2;
+    static constexpr int NBLK = (NS + SB - 1) / SB;
     static constexpr __host__ __device__ int uoff(int k) { return k * WIDTH - k * (k - 1) / 2; }
 };
 
@@ -378,8 +382,8 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     // ---- LU with partial pivoting, forward substitution in the RHS columns.
     // Rows slide one place per step (row[j] <- row[j+1] - l u[j+1]), so the
     // pivot column is always register 0 and every register index is static.
-    // Steps run as a real loop in blocks of kStepBlock with a static width that
-    // shrinks per block: compact code, at most kStepBlock-1 stale columns of
+    // Steps run as a real loop in blocks of SB steps with a static width that
+    // shrinks per block: compact code, at most SB-1 stale columns of
     // extra work per step.
     uint32_t done = 0;
 #pragma unroll
@@ -417,8 +421,8 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     int k = 0;
     for_blocks<0, T::NBLK>([&](auto bc) {
         constexpr int b = decltype(bc)::value;
-        constexpr int WB = WIDTH - kStepBlock * b;  // >= WIDTH - k for every k of the block
-        const int kend = NS < kStepBlock * (b + 1) ? NS : kStepBlock * (b + 1);
+        constexpr int WB = WIDTH - T::SB * b;  // >= WIDTH - k for every k of the block
+        const int kend = NS < T::SB * (b + 1) ? NS : T::SB * (b + 1);
 #pragma nounroll
         for (; k < kend; ++k) {
             const int v = 127 - (int)(kmax & 127u);
