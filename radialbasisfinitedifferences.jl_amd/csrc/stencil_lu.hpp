@@ -48,6 +48,18 @@ __device__ __forceinline__ double fma3(double a, double b, double c)
     return d;
 }
 
+// Lane `L` of each 16-lane DPP row, in every lane of that row (64-bit DPP
+// move, row_newbcast).  A DPP read needs two wait states after a VALU write
+// of its source, and the compiler does not see inside the asm: the s_nop
+// provides them whatever wrote the source.
+template <int L>
+__device__ __forceinline__ double row_bcast64(double v)
+{
+    double d;
+    asm("s_nop 1\n\tv_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf" : "=v"(d) : "v"(v), "i"(L));
+    return d;
+}
+
 template <int B, int NB, class F>
 __device__ __forceinline__ void for_blocks(F &&f)
 {
@@ -183,9 +195,9 @@ struct LuShape {
     // Phi(a, b), a <= b, at PHI0 + tri(a) + b
     static constexpr __host__ __device__ int tri(int a) { return a * (2 * NN - a + 1) / 2 - a; }
     static constexpr int SB = step_block<G>();
-    // +SB: a step writes up to SB-1 stale values past its row;
-    // +2 keeps the groups' slots off one LDS bank
-    static constexpr int USTRIDE = cmax_<UTOT, ASM>::v + SB + 2;
+    // +SB+16: a step writes up to SB-1 stale values past its row and reads
+    // its row in whole runs of 16; +2 keeps the groups' slots off one LDS bank
+    static constexpr int USTRIDE = cmax_<UTOT, ASM>::v + SB + 16 + 2;
     static constexpr int NBLK = (NS + SB - 1) / SB;
     static constexpr __host__ __device__ int uoff(int k) { return k * WIDTH - k * (k - 1) / 2; }
 };
@@ -442,6 +454,12 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             __builtin_amdgcn_wave_barrier();
             const double rp = L.rp[grp];
             singular |= (piv[0] == 0.0);
+            // the pivot row spread over each 16-lane DPP row: lane q holds
+            // columns q, q+16, ...; a DPP broadcast hands column j to the FMAs
+            constexpr int NPV = (WB + 15) / 16;
+            double pv[NPV];
+#pragma unroll
+            for (int i = 0; i < NPV; ++i) pv[i] = piv[(lane & 15) + 16 * i];
             double nl[ROWS];
 #pragma unroll
             for (int rr = 0; rr < ROWS; ++rr) nl[rr] = ((done >> rr) & 1u) ? -0.0 : -(row[rr][0] * rp);
@@ -453,12 +471,12 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             }
             candidate();
             kmax = group_max_u32<G>(key);
-#pragma unroll
-            for (int j = 1; j + 1 < WB; ++j) {
-                const double pj = piv[j + 1];
+            for_blocks<1, WB - 1>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const double pj = row_bcast64<(j + 1) % 16>(pv[(j + 1) / 16]);
 #pragma unroll
                 for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], pj, row[rr][j + 1]);
-            }
+            });
             // the next step's pivot lane overwrites the stale tail read here
             __builtin_amdgcn_wave_barrier();
         }
