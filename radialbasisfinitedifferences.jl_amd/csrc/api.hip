@@ -372,6 +372,10 @@ void generate_impl(rbffd_context *c, const double *X, long N, const double *Y, l
     if (m3) { Y = X; M = N; }
     if (n < 1 || n > N) throw Error{RBFFD_ERR_ARG, "stencil size must be in 1..number of X nodes"};
     if (m3 && bnd) throw Error{RBFFD_ERR_ARG, "boundary sets need Y"};
+    if (M == 0) {  // no Y nodes: the reference's loops run zero times
+        if (stats) *stats = rbffd_stats{};
+        return;
+    }
     for (int e = 0; e < 8; ++e)
         if (!c->ev[e]) RBFFD_HIP(hipEventCreate(&c->ev[e]));
     RBFFD_HIP(hipEventRecord(c->ev[0], s));
@@ -452,6 +456,12 @@ void csc_impl(rbffd_context *c, long M, int n, long ncols, const int *colidx, in
     hipStream_t s = c->stream;
     const long nnz = M * (long)n;
     if (ncols < 1) throw Error{RBFFD_ERR_ARG, "csc: ncols must be >= 1"};
+    if (nnz == 0) {  // sparse() of empty vectors: no columns referenced
+        rbffd::launch_fill_i64(colptr, ncols + 1, base, s);
+        if (ncols_used) *ncols_used = 0;
+        RBFFD_HIP(hipStreamSynchronize(s));
+        return;
+    }
     int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
     RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 2, s));
     max_kernel<<<nblk(nnz), 256, 0, s>>>(colidx, nnz, flag);

@@ -42,8 +42,9 @@ __host__ __device__ inline long grid_max_cells(long N) { return 4 * N + 64; }
 // Grid parameters computed on the device from box (see launch_bbox); bit 0 of
 // *flag (may be NULL) is set when the box is not finite.
 void launch_grid_params(const double *box, long N, int k, GridParams *out, int *flag, hipStream_t s);
-// p[0 .. n) = v; a kernel, so it can be recorded into a graph.
+// p[0 .. n) = v; kernels, so they can be recorded into a graph.
 void launch_fill_i32(int *p, long n, int v, hipStream_t s);
+void launch_fill_i64(int64_t *p, long n, int64_t v, hipStream_t s);
 // Exclusive scan of counts (ncell) into start (ncell + 1); tmp as returned by scan_tmp_bytes.
 size_t scan_tmp_bytes(long n);
 void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tmp_bytes, hipStream_t s);

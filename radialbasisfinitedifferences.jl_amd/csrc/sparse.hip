@@ -126,6 +126,18 @@ void launch_f64_to_f32(const double *src, long n, float *dst, hipStream_t s)
     f64_to_f32_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, n, dst);
 }
 
+__global__ void fill_i64_kernel(int64_t *p, long n, int64_t v)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_i64(int64_t *p, long n, int64_t v, hipStream_t s)
+{
+    if (n <= 0) return;
+    fill_i64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+}
+
 // ------------------------------------------------------------------ SpMV/SpMM
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v)
