@@ -5,10 +5,16 @@
 #include <string.h>
 
 #include <algorithm>
+#include <climits>
+#include <condition_variable>
 #include <cstdlib>
 #include <exception>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -34,11 +40,66 @@ struct rbffd_arena {
 
 constexpr int kLanes = 4;  // parallel branches of a captured batch
 
+// A few persistent host threads that run one job at a time (the host side of
+// the staged device-to-host copies).
+struct rbffd_host_pool {
+    std::vector<std::thread> threads;
+    std::mutex mu;
+    std::condition_variable wake, done;
+    std::function<void(int, int)> job;
+    long generation = 0;
+    int pending = 0;
+    bool stop = false;
+
+    explicit rbffd_host_pool(int n)
+    {
+        for (int t = 0; t < n; ++t) threads.emplace_back([this, t, n] { loop(t, n); });
+    }
+    ~rbffd_host_pool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        wake.notify_all();
+        for (auto &th : threads) th.join();
+    }
+    void loop(int t, int n)
+    {
+        long seen = 0;
+        for (;;) {
+            std::function<void(int, int)> f;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                wake.wait(lk, [&] { return stop || generation != seen; });
+                if (stop) return;
+                seen = generation;
+                f = job;
+            }
+            f(t, n);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (--pending == 0) done.notify_all();
+            }
+        }
+    }
+    // run f(thread, threads) on every thread and wait for all of them
+    void run(std::function<void(int, int)> f)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        job = std::move(f);
+        pending = (int)threads.size();
+        ++generation;
+        wake.notify_all();
+        done.wait(lk, [&] { return pending == 0; });
+    }
+};
+
 // Workspace slots of the context (one grow-only device buffer each).
 enum Slot {
     S_BBOX_PART, S_BBOX, S_FLAG, S_XGROUP, S_QGROUP, S_STENCILS, S_Y2X, S_STATUS, S_ROWS, S_CSC_COUNTS,
     S_CSC_PTR, S_CSC_FILL, S_CSC_PERM, S_CSC_ROW, S_H_X, S_H_Y, S_H_COL, S_H_VALS, S_H_CVALS,
-    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS, S_MORTON_KEY, S_MORTON_IDX, S_NSLOT
+    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS, S_MORTON_KEY, S_MORTON_IDX, S_CSC_PERM_IN, S_NSLOT
 };
 
 struct rbffd_context {
@@ -61,6 +122,10 @@ struct rbffd_context {
     std::vector<int64_t> batch_key;
     hipGraph_t batch_graph = nullptr;
     hipGraphExec_t batch_exec = nullptr;
+    // staged device-to-host copies: two pinned buffers and host copy threads
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    std::unique_ptr<rbffd_host_pool> pool;
 };
 
 namespace {
@@ -118,10 +183,20 @@ __global__ void count_negative_kernel(const int *v, long n, int *flag)
     if (i < n && v[i] < 0) atomicAdd(flag, 1);
 }
 
-__global__ void max_kernel(const int *v, long n, int *out)
+// Largest value: a block maximum, then one atomic per block.
+__global__ __launch_bounds__(256) void max_kernel(const int *v, long n, int *out)
 {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < n) atomicMax(out, v[i]);
+    __shared__ int sm[256];
+    int m = INT_MIN;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        m = max(m, v[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, sm[0]);
 }
 
 __global__ void status_count_kernel(const int *status, long n, int *counts)
@@ -147,6 +222,48 @@ int read_flag(rbffd_context *c, const int *dflag)
     RBFFD_HIP(hipMemcpyAsync(&h, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     RBFFD_HIP(hipStreamSynchronize(c->stream));
     return h;
+}
+
+// Device-to-host copy into pageable caller memory, pipelined: chunks go over
+// PCIe into two pinned buffers in turn while the host threads copy the
+// previous chunk into place.  Synchronous.
+constexpr size_t kStageBytes = 32u << 20;
+
+void d2h_staged(rbffd_context *c, void *dst, const void *src, size_t bytes)
+{
+    hipStream_t s = c->stream;
+    if (bytes < 2 * kStageBytes) {
+        RBFFD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        RBFFD_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    for (int b = 0; b < 2; ++b) {
+        if (!c->stage[b]) RBFFD_HIP(hipHostMalloc(&c->stage[b], kStageBytes, hipHostMallocDefault));
+        if (!c->stage_ev[b]) RBFFD_HIP(hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming));
+    }
+    if (!c->pool) {
+        const unsigned hw = std::thread::hardware_concurrency();
+        c->pool = std::make_unique<rbffd_host_pool>((int)std::max(1u, std::min(8u, hw)));
+    }
+    const size_t nchunk = (bytes + kStageBytes - 1) / kStageBytes;
+    auto enqueue = [&](size_t i) {
+        const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        RBFFD_HIP(hipMemcpyAsync(c->stage[i & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, s));
+        RBFFD_HIP(hipEventRecord(c->stage_ev[i & 1], s));
+    };
+    enqueue(0);
+    for (size_t i = 0; i < nchunk; ++i) {
+        // chunk i+1 reuses the buffer whose host copy finished in the previous round
+        if (i + 1 < nchunk) enqueue(i + 1);
+        RBFFD_HIP(hipEventSynchronize(c->stage_ev[i & 1]));
+        const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        const char *from = (const char *)c->stage[i & 1];
+        char *to = (char *)dst + off;
+        c->pool->run([&](int t, int n) {
+            const size_t part = (len + n - 1) / n, a = std::min(len, part * t), b = std::min(len, a + part);
+            if (b > a) memcpy(to + a, from + a, b - a);
+        });
+    }
 }
 
 // ------------------------------------------------------------ kNN
@@ -464,7 +581,7 @@ void csc_impl(rbffd_context *c, long M, int n, long ncols, const int *colidx, in
     }
     int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
     RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 2, s));
-    max_kernel<<<nblk(nnz), 256, 0, s>>>(colidx, nnz, flag);
+    max_kernel<<<(unsigned)std::min<long>(2048, nblk(nnz)), 256, 0, s>>>(colidx, nnz, flag);
     count_negative_kernel<<<nblk(nnz), 256, 0, s>>>(colidx, nnz, flag + 1);
     int hm[2] = {0, 0};
     RBFFD_HIP(hipMemcpyAsync(hm, flag, sizeof(hm), hipMemcpyDeviceToHost, s));
@@ -483,8 +600,9 @@ void csc_impl(rbffd_context *c, long M, int n, long ncols, const int *colidx, in
     size_t tb = rbffd::scan_tmp_bytes(ncols + 1);
     void *tmp = ws(c, S_SCAN_TMP2, tb);
     rbffd::launch_exclusive_scan(counts, ptr, ncols + 1, tmp, tb, s);
-    rbffd::launch_csc_scatter(colidx, M, n, ptr, fill, perm, row, s);
-    rbffd::launch_csc_sort_columns(ptr, ncols, row, perm, s);
+    int *perm_in = (int *)ws(c, S_CSC_PERM_IN, sizeof(int) * nnz);
+    rbffd::launch_csc_scatter(colidx, M, n, ptr, fill, perm_in, s);
+    rbffd::launch_csc_rank(colidx, M, n, ptr, perm_in, perm, row, s);
     for (int o = 0; o < nops; ++o) rbffd::launch_gather_f64(csr_vals[o], perm, nnz, csc_vals[o], s);
     rbffd::launch_i32_to_i64(ptr, ncols + 1, colptr, base, s);
     rbffd::launch_i32_to_i64(row, nnz, rowval, base, s);
@@ -692,6 +810,11 @@ int rbffd_destroy(rbffd_context *c)
         for (auto &e : c->ev)
             if (e) (void)hipEventDestroy(e);
         for (auto &e : c->tune_ev)
+            if (e) (void)hipEventDestroy(e);
+        c->pool.reset();
+        for (auto &b : c->stage)
+            if (b) (void)hipHostFree(b);
+        for (auto &e : c->stage_ev)
             if (e) (void)hipEventDestroy(e);
         if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
         for (auto &e : c->join_ev)
@@ -915,10 +1038,9 @@ int rbffd_generate_operator(rbffd_context *c, const double *X, int64_t N, const 
         csc_impl(c, Mr, n, N, dcol, nops, vp, dptr, drow, cvp, index_base, &used);
         if (ncols) *ncols = used;
         RBFFD_HIP(hipMemcpyAsync(colptr, dptr, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost, s));
-        RBFFD_HIP(hipMemcpyAsync(rowval, drow, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost, s));
-        for (int o = 0; o < nops; ++o)
-            RBFFD_HIP(hipMemcpyAsync(nzval[o], cvp[o], sizeof(double) * nnz, hipMemcpyDeviceToHost, s));
         RBFFD_HIP(hipStreamSynchronize(s));
+        d2h_staged(c, rowval, drow, sizeof(int64_t) * nnz);
+        for (int o = 0; o < nops; ++o) d2h_staged(c, nzval[o], cvp[o], sizeof(double) * nnz);
     });
 }
 

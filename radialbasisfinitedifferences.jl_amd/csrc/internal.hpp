@@ -100,9 +100,9 @@ RBFFD_SHAPES(RBFFD_DECLARE_SHAPE)
 // --------------------------------------------------- sparse (K3, K4)
 void launch_fill_rowptr(int64_t *rowptr, long M, int n, hipStream_t s);
 void launch_csc_count(const int *colidx, long nnz, int *counts, hipStream_t s);
-void launch_csc_scatter(const int *colidx, long M, int n, const int *colptr, int *fill, int *perm,
-                        int *rowval, hipStream_t s);
-void launch_csc_sort_columns(const int *colptr, long ncols, int *rowval, int *perm, hipStream_t s);
+void launch_csc_scatter(const int *colidx, long M, int n, const int *colptr, int *fill, int *perm, hipStream_t s);
+void launch_csc_rank(const int *colidx, long M, int n, const int *colptr, const int *perm_in, int *perm,
+                     int *rowval, hipStream_t s);
 void launch_gather_f64(const double *src, const int *perm, long nnz, double *dst, hipStream_t s);
 void launch_i32_to_i64(const int *src, long n, int64_t *dst, int64_t add, hipStream_t s);
 

@@ -44,50 +44,46 @@ void launch_csc_count(const int *colidx, long nnz, int *counts, hipStream_t s)
     csc_count_kernel<<<(unsigned)((nnz + 255) / 256), 256, 0, s>>>(colidx, nnz, counts);
 }
 
-__global__ void csc_scatter_kernel(const int *colidx, long M, int n, const int *colptr, int *fill,
-                                   int *perm, int *rowval)
+// Entries of each column, in arbitrary order (atomic slots).
+__global__ void csc_scatter_kernel(const int *colidx, long nnz, const int *colptr, int *fill, int *perm)
 {
     long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (p >= M * (long)n) return;
+    if (p >= nnz) return;
     int c = colidx[p];
     int pos = colptr[c] + atomicAdd(&fill[c], 1);
     perm[pos] = (int)p;
-    rowval[pos] = (int)(p / n);
 }
 
-void launch_csc_scatter(const int *colidx, long M, int n, const int *colptr, int *fill, int *perm,
-                        int *rowval, hipStream_t s)
+void launch_csc_scatter(const int *colidx, long M, int n, const int *colptr, int *fill, int *perm, hipStream_t s)
 {
     long nnz = M * (long)n;
     if (nnz <= 0) return;
-    csc_scatter_kernel<<<(unsigned)((nnz + 255) / 256), 256, 0, s>>>(colidx, M, n, colptr, fill, perm,
-                                                                      rowval);
+    csc_scatter_kernel<<<(unsigned)((nnz + 255) / 256), 256, 0, s>>>(colidx, nnz, colptr, fill, perm);
 }
 
-// One thread per column: insertion sort of its (row, perm) pairs.  Columns hold
-// a few dozen entries, rows inside a column are distinct.
-__global__ void csc_sort_kernel(const int *colptr, long ncols, int *rowval, int *perm)
+// Final order inside each column: an entry's slot is the number of entries of
+// its column with a smaller entry index (entry p lies in row p / n, so rows
+// come out ascending).  One thread per entry counts over its column's
+// unsorted segment; columns hold a few dozen entries.
+__global__ void csc_rank_kernel(const int *colidx, long nnz, int n, const int *colptr, const int *perm_in,
+                                int *perm, int *rowval)
 {
-    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    const int c = colidx[p];
     const int b = colptr[c], e = colptr[c + 1];
-    for (int i = b + 1; i < e; ++i) {
-        const int r = rowval[i], q = perm[i];
-        int j = i - 1;
-        while (j >= b && rowval[j] > r) {
-            rowval[j + 1] = rowval[j];
-            perm[j + 1] = perm[j];
-            --j;
-        }
-        rowval[j + 1] = r;
-        perm[j + 1] = q;
-    }
+    int rank = 0;
+    for (int i = b; i < e; ++i) rank += perm_in[i] < (int)p ? 1 : 0;
+    perm[b + rank] = (int)p;
+    rowval[b + rank] = (int)(p / n);
 }
 
-void launch_csc_sort_columns(const int *colptr, long ncols, int *rowval, int *perm, hipStream_t s)
+void launch_csc_rank(const int *colidx, long M, int n, const int *colptr, const int *perm_in, int *perm,
+                     int *rowval, hipStream_t s)
 {
-    if (ncols <= 0) return;
-    csc_sort_kernel<<<(unsigned)((ncols + 255) / 256), 256, 0, s>>>(colptr, ncols, rowval, perm);
+    const long nnz = M * (long)n;
+    if (nnz <= 0) return;
+    csc_rank_kernel<<<(unsigned)((nnz + 255) / 256), 256, 0, s>>>(colidx, nnz, n, colptr, perm_in, perm, rowval);
 }
 
 __global__ void gather_f64_kernel(const double *src, const int *perm, long nnz, double *dst)

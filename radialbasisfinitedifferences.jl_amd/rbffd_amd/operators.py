@@ -22,6 +22,7 @@ where Julia raises ArgumentError (e.g. n larger than the node count).
 from __future__ import annotations
 
 import ctypes
+import time
 
 import numpy as np
 import scipy.sparse as sp
@@ -115,15 +116,19 @@ def operators(X, Y, p, n, polydeg, ops, k=0, boundary=None, precision="fp64", ct
     Y_ptr = None
     if Yp is not None:
         Y_ptr = _vp(X) if Y is X else _vp(Yp)
+    t0 = time.perf_counter()
     code = L.rbffd_generate_operator(ctx.handle, _vp(X), N, Y_ptr, M, int(p), n, int(polydeg),
                                      ctypes.byref(bnd.s) if bnd else None, nops, _vp(kinds), _vp(ks),
                                      PREC[precision], 0, _vp(colptr), _vp(rowval),
                                      ctypes.cast(nzp, ctypes.c_void_p), ctypes.byref(ncols), ctypes.byref(stats))
+    t_call = time.perf_counter() - t0
     check(code)
     nc = int(ncols.value)
     mats = tuple(sp.csc_matrix((nz[o], rowval, colptr[: nc + 1]), shape=(M, nc)) for o in range(nops))
     if return_stats:
-        return mats, stats.as_dict()
+        st = stats.as_dict()
+        st["host_call_s"] = t_call  # the C entry point alone: H2D, kernels, D2H
+        return mats, st
     return mats
 
 
