@@ -249,7 +249,7 @@ def main():
         del col, vals, st_d, Xd
         extras = {}
         for name, fn in (("hyperviscosity_4m", extra_hyper4m), ("uneven_batch_64", extra_batch),
-                         ("advdiff_ssprk3_4m", extra_advdiff)):
+                         ("advdiff_ssprk3_4m", extra_advdiff), ("poisson_solve", extra_poisson_solve)):
             try:
                 extras[name] = fn(rbffd_amd, ctx, eng, args.no_cpu)
             except Exception as e:  # a failed extra is reported, the headline stands
@@ -412,6 +412,73 @@ def extra_advdiff(rbffd_amd, ctx, eng, no_cpu):
     ref = finals["fp64"]
     res["fp32_values_final_rel_err"] = float(np.abs(finals["fp32_values"] - ref).max() / np.abs(ref).max())
     res["final_max_abs"] = float(np.abs(ref).max())
+    return res
+
+
+def extra_poisson_solve(rbffd_amd, ctx, eng, no_cpu):
+    """`D \\ f` of the Poisson harness on the GPU (rbffd_lsq_solve_device).
+    Square: the 51 x 51 Dirichlet system (r^5, degree 2, n = 13) by QR.
+    Oversampled: X the same 51 x 51 nodes, Y a 101 x 101 jittered grid,
+    Laplacian rows inside and E rows on the boundary, scaled as the reference
+    scales its blocks; QR and CGLS (tol 1e-12).  Times are device time of the
+    whole call (checks, densify or transpose, solve)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    from rbffd_amd.nodes import config_nodes, jittered_grid
+
+    log("extra: Poisson harness solves (square 2601, oversampled 10201 x 2601)")
+    X, prm = config_nodes("poisson2500")
+    u = lambda P: np.sin(np.pi * P[:, 0]) * np.sin(np.pi * P[:, 1])
+    edge = lambda P: (P[:, 0] <= 0) | (P[:, 0] >= 1) | (P[:, 1] <= 0) | (P[:, 1] >= 1)
+
+    def timed(A, b, method):
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        dev = [ctx.to_device(a) for a in (A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, b)]
+        x = ctx.zeros((A.shape[1],), np.float64)
+        eng.lsq_solve(*dev, x, method=method, tol=1e-12, max_iters=100000)   # warm-up (handles, workspace)
+        x.zero_()
+        ctx.synchronize()
+        ctx.timer_start()
+        its, rel = eng.lsq_solve(*dev, x, method=method, tol=1e-12, max_iters=100000)
+        ms = ctx.timer_stop()
+        return x.get(), ms, its, rel
+
+    res = {}
+    E, Lap = rbffd_amd.operators(X, X, prm["p"], prm["n"], prm["polydeg"], ["E", "Lap"], ctx=ctx)
+    ed = np.where(edge(X))[0]
+    A = sp.lil_matrix(Lap.tocsr())
+    A[ed, :] = E.tocsr()[ed, :]
+    A = A.tocsr()
+    b = -2 * np.pi**2 * u(X)
+    b[ed] = u(X)[ed]
+    x, ms, _, _ = timed(A, b, "qr")
+    t0 = time.perf_counter()
+    ref = sla.spsolve(A.tocsc(), b)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    res["square_2601_qr"] = {"ms": ms, "rel_l2_error": float(np.linalg.norm(x - u(X)) / np.linalg.norm(u(X))),
+                             "vs_splu": float(np.linalg.norm(x - ref) / np.linalg.norm(ref)),
+                             "cpu_splu_ms": cpu_ms}
+
+    Y = jittered_grid(101, 101, 0.25, seed=11)
+    E, Lap = rbffd_amd.operators(X, Y, prm["p"], prm["n"], prm["polydeg"], ["E", "Lap"], ctx=ctx)
+    yb = edge(Y)
+    yi = np.where(~yb)[0]
+    yd = np.where(yb)[0]
+    d2 = np.sort(((X[:, None, :] - X[None, :, :]) ** 2).sum(-1), axis=1)[:, 1]
+    h = float(np.mean(np.sqrt(d2)))
+    A = sp.vstack([Lap.tocsr()[yi] / np.sqrt(len(yi)), E.tocsr()[yd] / (h * np.sqrt(len(yd)))]).tocsr()
+    b = np.concatenate([-2 * np.pi**2 * u(Y[yi]) / np.sqrt(len(yi)), u(Y[yd]) / (h * np.sqrt(len(yd)))])
+    t0 = time.perf_counter()
+    ref = np.linalg.lstsq(A.toarray(), b, rcond=None)[0]
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    for method in ("qr", "cgls"):
+        x, ms, its, rel = timed(A, b, method)
+        res[f"oversampled_10201x2601_{method}"] = {
+            "ms": ms, "iterations": its, "relres": rel,
+            "rel_l2_error": float(np.linalg.norm(x - u(X)) / np.linalg.norm(u(X))),
+            "vs_lstsq": float(np.linalg.norm(x - ref) / np.linalg.norm(ref))}
+    res["oversampled_cpu_dense_lstsq_ms"] = cpu_ms
     return res
 
 

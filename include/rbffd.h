@@ -202,6 +202,34 @@ int rbffd_locality_order_device(rbffd_context *ctx, const double *X, int64_t N, 
 int rbffd_combine_device(rbffd_context *ctx, int64_t nnz, int32_t nops, const double *const *vals,
                          const double *coef, double *out, float *out32);
 
+/* ------------------------------------------------------ sparse solve
+ * x = argmin ||D x - b||_2 for a sparse D (M x N, M >= N; M = N solves the
+ * square system), CSR with int64 rowptr, int32 columns and FP64 values, all
+ * device pointers.  Replaces `D \ f` of the reference's PDE harness
+ * (test/poisson_test.jl:121, test/mesh_import_test.jl:150) on the
+ * collocation matrix assembled at test/poisson_test.jl:75-113, where Julia
+ * runs a sparse QR (M > N) or LU (M = N) factorisation.
+ *   RBFFD_SOLVE_QR:   D densified on the device and solved by rocSOLVER's QR
+ *                     least squares (dgels; librocsolver.so.0 is opened at
+ *                     run time, RBFFD_ERR_UNSUPPORTED without it).  Needs
+ *                     M N < 2^31; a rank-deficient D gives RBFFD_ERR_SINGULAR.
+ *                     The start value of x is not used.
+ *   RBFFD_SOLVE_CGLS: conjugate gradients on the column-equilibrated normal
+ *                     equations, every iteration on the device; x holds the
+ *                     start (zeros for none).  Stops once
+ *                     ||C D^T (b - D x)|| <= tol ||C D^T (b - D x0)|| (C the
+ *                     column scaling) or after max_iters iterations.
+ *   RBFFD_SOLVE_AUTO: QR when the dense copy has at most 2^28 entries and
+ *                     rocSOLVER loads, CGLS otherwise.
+ * *iters and *relres (each may be NULL) receive the CGLS iteration count and
+ * the ratio above (0 and 0 for QR).  Needs M, N, nnz < 2^31.  Synchronises. */
+#define RBFFD_SOLVE_AUTO 0
+#define RBFFD_SOLVE_CGLS 1
+#define RBFFD_SOLVE_QR 2
+int rbffd_lsq_solve_device(rbffd_context *ctx, int64_t M, int64_t N, const int64_t *rowptr, const int32_t *col,
+                           const double *vals, const double *b, double *x, int32_t method, double tol,
+                           int64_t max_iters, int64_t *iters, double *relres);
+
 /* ----------------------------------------------------- time stepping
  * `steps` steps of the three-stage strong-stability-preserving Runge-Kutta
  * method (Shu-Osher form) for du/dt = L u on F fields stored node-major, each

@@ -99,7 +99,8 @@ struct rbffd_host_pool {
 enum Slot {
     S_BBOX_PART, S_BBOX, S_FLAG, S_XGROUP, S_QGROUP, S_STENCILS, S_Y2X, S_STATUS, S_ROWS, S_CSC_COUNTS,
     S_CSC_PTR, S_CSC_FILL, S_CSC_PERM, S_CSC_ROW, S_H_X, S_H_Y, S_H_COL, S_H_VALS, S_H_CVALS,
-    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS, S_MORTON_KEY, S_MORTON_IDX, S_CSC_PERM_IN, S_NSLOT
+    S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS, S_MORTON_KEY, S_MORTON_IDX, S_CSC_PERM_IN, S_LSQ_ROWOF, S_LSQ_TCOL, S_LSQ_TPTR, S_LSQ_F64,
+    S_LSQ_COUNT, S_LSQ_DENSE, S_LSQ_INFO, S_NSLOT
 };
 
 struct rbffd_context {
@@ -126,6 +127,8 @@ struct rbffd_context {
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     std::unique_ptr<rbffd_host_pool> pool;
+    // rocSOLVER handle of the direct sparse solve (created on first use)
+    void *solver = nullptr;
 };
 
 namespace {
@@ -610,6 +613,180 @@ void csc_impl(rbffd_context *c, long M, int n, long ncols, const int *colidx, in
     RBFFD_HIP(hipStreamSynchronize(s));
 }
 
+// ------------------------------------------------------------ sparse solve
+// rows whose rowptr decreases
+__global__ void count_decreasing_kernel(const int64_t *rowptr, long M, int *flag)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < M && rowptr[i + 1] < rowptr[i]) atomicAdd(flag, 1);
+}
+
+// Iterations per recorded graph.  Even: the two gamma slots alternate by
+// iteration parity, and every replay starts at an even iteration.
+constexpr long kLsqChunk = 32;
+
+struct LsqState {
+    long M, N, nnz;
+    const int64_t *rowptr;
+    const int *col;
+    const double *dval;  // D C
+    const int64_t *tptr;
+    const int *tcol;
+    const double *tval;  // (D C)^T
+    double *scale, *z, *p, *s, *r, *q;
+    double *g;           // g[0], g[1]: gamma by iteration parity; g[2]: |q|^2
+    double *partial;
+    unsigned *count;
+};
+
+// CGLS iteration k (its gamma is g[k & 1]; the next one goes to g[(k + 1) & 1])
+void lsq_iteration(const LsqState &S, long k, hipStream_t st)
+{
+    const double *gam = S.g + (k & 1);
+    double *gam_new = S.g + ((k + 1) & 1);
+    rbffd::launch_lsq_spmv(S.M, S.nnz, S.rowptr, S.col, S.dval, S.p, S.q, {S.partial, S.count, S.g + 2}, st);
+    rbffd::launch_lsq_update(S.N, S.M, gam, S.g + 2, S.p, S.q, S.z, S.r, st);
+    rbffd::launch_lsq_spmv(S.N, S.nnz, S.tptr, S.tcol, S.tval, S.r, S.s, {S.partial, S.count, gam_new}, st);
+    rbffd::launch_lsq_direction(S.N, gam, gam_new, S.s, S.p, st);
+}
+
+// Largest dense copy (entries) the automatic choice sends to the direct method.
+constexpr long kDenseAutoEntries = 1L << 28;
+
+void lsq_impl(rbffd_context *c, long M, long N, const int64_t *rowptr, const int *col, const double *vals,
+              const double *b, double *x, int method, double tol, long max_iters, int64_t *iters_out,
+              double *relres_out)
+{
+    hipStream_t st = c->stream;
+    if (iters_out) *iters_out = 0;
+    if (relres_out) *relres_out = 0.0;
+    if (M < N) throw Error{RBFFD_ERR_ARG, "lsq: D needs at least as many rows as columns"};
+    if (M > INT_MAX || N > INT_MAX) throw Error{RBFFD_ERR_ARG, "lsq: more than 2^31 - 1 rows"};
+    if (!(tol >= 0.0) || max_iters < 0) throw Error{RBFFD_ERR_ARG, "lsq: tol and max_iters must be >= 0"};
+    if (N == 0) return;
+    int64_t ends[2] = {0, 0};
+    RBFFD_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RBFFD_HIP(hipMemcpyAsync(&ends[1], rowptr + M, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RBFFD_HIP(hipStreamSynchronize(st));
+    const long nnz = (long)ends[1];
+    if (ends[0] != 0 || nnz < 0) throw Error{RBFFD_ERR_ARG, "lsq: rowptr must start at 0 and end at nnz"};
+    if (nnz > INT_MAX) throw Error{RBFFD_ERR_ARG, "lsq: more than 2^31 - 1 entries"};
+    if (nnz == 0) return;  // D = 0: every x is a least-squares solution; x stays the start
+    int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
+    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 3, st));
+    max_kernel<<<(unsigned)std::min<long>(2048, nblk(nnz)), 256, 0, st>>>(col, nnz, flag);
+    count_negative_kernel<<<nblk(nnz), 256, 0, st>>>(col, nnz, flag + 1);
+    count_decreasing_kernel<<<nblk(M), 256, 0, st>>>(rowptr, M, flag + 2);
+    int hf[3] = {0, 0, 0};
+    RBFFD_HIP(hipMemcpyAsync(hf, flag, sizeof(hf), hipMemcpyDeviceToHost, st));
+    RBFFD_HIP(hipStreamSynchronize(st));
+    if (hf[1] > 0 || hf[0] >= N) throw Error{RBFFD_ERR_ARG, "lsq: column index out of range"};
+    if (hf[2] > 0) throw Error{RBFFD_ERR_ARG, "lsq: rowptr decreases"};
+
+    if (method == RBFFD_SOLVE_AUTO)
+        method = (rbffd::dense_solver_available() && M * N <= kDenseAutoEntries) ? RBFFD_SOLVE_QR : RBFFD_SOLVE_CGLS;
+    if (method == RBFFD_SOLVE_QR) {
+        if (M * N > (long)INT_MAX) throw Error{RBFFD_ERR_ARG, "lsq: the direct method needs M N < 2^31"};
+        double *A = (double *)ws(c, S_LSQ_DENSE, sizeof(double) * ((size_t)M * N + M));
+        int *info = (int *)ws(c, S_LSQ_INFO, sizeof(int) * 4);
+        const int r = rbffd::dense_lsq(&c->solver, M, N, rowptr, col, vals, b, x, A, A + (size_t)M * N, info, st);
+        if (r > 0) throw Error{RBFFD_ERR_SINGULAR, "lsq: D is rank deficient (zero diagonal " + std::to_string(r) + " in R)"};
+        return;
+    }
+
+    // the transpose as its own CSR, rows ascending inside each transposed row
+    int *row_of = (int *)ws(c, S_LSQ_ROWOF, sizeof(int) * nnz);
+    int *counts = (int *)ws(c, S_CSC_COUNTS, sizeof(int) * (N + 1));
+    int *ptr = (int *)ws(c, S_CSC_PTR, sizeof(int) * (N + 1));
+    int *fill = (int *)ws(c, S_CSC_FILL, sizeof(int) * N);
+    int *perm_in = (int *)ws(c, S_CSC_PERM_IN, sizeof(int) * nnz);
+    int *perm = (int *)ws(c, S_CSC_PERM, sizeof(int) * nnz);
+    int *entry = (int *)ws(c, S_CSC_ROW, sizeof(int) * nnz);
+    int *tcol = (int *)ws(c, S_LSQ_TCOL, sizeof(int) * nnz);
+    int64_t *tptr = (int64_t *)ws(c, S_LSQ_TPTR, sizeof(int64_t) * (N + 1));
+    const size_t nf = 2 * (size_t)nnz + 4 * (size_t)N + 2 * (size_t)M + rbffd::kLsqBlocks + 4;
+    double *f = (double *)ws(c, S_LSQ_F64, sizeof(double) * nf);
+    unsigned *count = (unsigned *)ws(c, S_LSQ_COUNT, sizeof(unsigned) * 4);
+    LsqState S;
+    S.M = M; S.N = N; S.nnz = nnz; S.rowptr = rowptr; S.col = col;
+    double *dval = f, *tval = dval + nnz;
+    S.scale = tval + nnz; S.z = S.scale + N; S.p = S.z + N; S.s = S.p + N;
+    S.r = S.s + N; S.q = S.r + M; S.partial = S.q + M; S.g = S.partial + rbffd::kLsqBlocks;
+    S.dval = dval; S.tptr = tptr; S.tcol = tcol; S.tval = tval; S.count = count;
+
+    rbffd::launch_row_of(rowptr, M, row_of, st);
+    rbffd::launch_fill_i32(counts, N + 1, 0, st);
+    rbffd::launch_fill_i32(fill, N, 0, st);
+    rbffd::launch_fill_i32((int *)count, 4, 0, st);
+    rbffd::launch_csc_count(col, nnz, counts, st);
+    const size_t tb = rbffd::scan_tmp_bytes(N + 1);
+    void *tmp = ws(c, S_SCAN_TMP2, tb);
+    rbffd::launch_exclusive_scan(counts, ptr, N + 1, tmp, tb, st);
+    rbffd::launch_csc_scatter(col, nnz, 1, ptr, fill, perm_in, st);
+    rbffd::launch_csc_rank(col, nnz, 1, ptr, perm_in, perm, entry, st);
+    rbffd::launch_gather_i32(row_of, perm, nnz, tcol, st);
+    rbffd::launch_i32_to_i64(ptr, N + 1, tptr, 0, st);
+    rbffd::launch_gather_f64(vals, perm, nnz, tval, st);       // D^T
+    rbffd::launch_col_scale(tptr, N, tval, S.scale, st);
+    rbffd::launch_scale_cols(col, nnz, S.scale, vals, dval, st);
+    rbffd::launch_gather_f64(dval, perm, nnz, tval, st);       // (D C)^T
+
+    // r = b - D x0, z = 0, s = (D C)^T r, gamma0 = |s|^2, p = s
+    rbffd::launch_copy_f64(b, M, S.r, st);
+    rbffd::launch_spmv(M, rowptr, col, vals, 0, 1, x, S.r, -1.0, 1.0, 0, nullptr, st);
+    rbffd::launch_fill_f64(S.z, N, 0.0, st);
+    rbffd::launch_fill_f64(S.g, 4, 0.0, st);
+    rbffd::launch_lsq_spmv(N, nnz, tptr, tcol, tval, S.r, S.s, {S.partial, count, S.g}, st);
+    rbffd::launch_copy_f64(S.s, N, S.p, st);
+    RBFFD_HIP(hipGetLastError());
+    double g0 = 0.0;
+    RBFFD_HIP(hipMemcpyAsync(&g0, S.g, sizeof(double), hipMemcpyDeviceToHost, st));
+    RBFFD_HIP(hipStreamSynchronize(st));
+    if (!std::isfinite(g0)) throw Error{RBFFD_ERR_NONFINITE, "lsq: non-finite values in D, b or x"};
+
+    long done = 0;
+    double rel = 0.0;
+    if (g0 > 0.0 && max_iters > 0) {
+        rel = 1.0;
+        // a chunk of iterations recorded once and replayed (the stream must
+        // be capturable: not the null stream)
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (st && max_iters >= kLsqChunk) {
+            RBFFD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (long k = 0; k < kLsqChunk; ++k) lsq_iteration(S, k, st);
+            RBFFD_HIP(hipStreamEndCapture(st, &graph));
+            RBFFD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        try {
+            while (done < max_iters && rel > tol) {
+                const long k = std::min(kLsqChunk, max_iters - done);
+                if (exec && k == kLsqChunk)
+                    RBFFD_HIP(hipGraphLaunch(exec, st));
+                else
+                    for (long j = 0; j < k; ++j) lsq_iteration(S, done + j, st);
+                done += k;
+                double g = 0.0;
+                RBFFD_HIP(hipMemcpyAsync(&g, S.g + (done & 1), sizeof(double), hipMemcpyDeviceToHost, st));
+                RBFFD_HIP(hipStreamSynchronize(st));
+                if (!std::isfinite(g)) throw Error{RBFFD_ERR_NONFINITE, "lsq: non-finite iterate"};
+                rel = std::sqrt(g / g0);
+            }
+        } catch (...) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        if (exec) RBFFD_HIP(hipGraphExecDestroy(exec));
+        if (graph) RBFFD_HIP(hipGraphDestroy(graph));
+    }
+    rbffd::launch_lsq_finish(N, S.scale, S.z, x, st);
+    RBFFD_HIP(hipGetLastError());
+    RBFFD_HIP(hipStreamSynchronize(st));
+    if (iters_out) *iters_out = done;
+    if (relres_out) *relres_out = rel;
+}
+
 // ------------------------------------------------------------ batch
 // Per set: 8 flag words.  [0] bit 0: X not finite; [1] kNN result slots at -1;
 // [2 + s] rows with status s (1 singular, 2 non-finite weights).
@@ -800,6 +977,7 @@ int rbffd_destroy(rbffd_context *c)
             if (st) (void)hipStreamSynchronize(st);
         if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
         if (c->batch_graph) (void)hipGraphDestroy(c->batch_graph);
+        rbffd::dense_solver_destroy(c->solver);
         for (auto &b : c->bufs)
             if (b.p) (void)hipFree(b.p);
         for (auto &b : c->knn_arena.b)
@@ -981,6 +1159,19 @@ int rbffd_csr_to_csc_device(rbffd_context *c, int64_t M, int32_t n, int64_t ncol
             throw Error{RBFFD_ERR_ARG, "null pointer"};
         if (M < 1 || n < 1) throw Error{RBFFD_ERR_ARG, "empty matrix"};
         csc_impl(c, M, n, ncols, colidx, nops, csr_vals, colptr, rowval, csc_vals, index_base, ncols_used);
+    });
+}
+
+int rbffd_lsq_solve_device(rbffd_context *c, int64_t M, int64_t N, const int64_t *rowptr, const int32_t *col,
+                           const double *vals, const double *b, double *x, int32_t method, double tol,
+                           int64_t max_iters, int64_t *iters, double *relres)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (M < 0 || N < 0) throw Error{RBFFD_ERR_ARG, "negative size"};
+        if (N > 0 && (!rowptr || !col || !vals || !b || !x)) throw Error{RBFFD_ERR_ARG, "null pointer"};
+        if (method < RBFFD_SOLVE_AUTO || method > RBFFD_SOLVE_QR) throw Error{RBFFD_ERR_ARG, "unknown solve method"};
+        lsq_impl(c, M, N, rowptr, col, vals, b, x, method, tol, max_iters, iters, relres);
     });
 }
 

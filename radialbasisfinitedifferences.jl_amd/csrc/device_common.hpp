@@ -70,6 +70,32 @@ __device__ __forceinline__ double bcast_f64(double v, int src)
     return *reinterpret_cast<double *>(&p);
 }
 
+// ---- sums over lane groups (SpMV rows) ------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    int2 p = *reinterpret_cast<int2 *>(&v);
+    p.x = __builtin_amdgcn_update_dpp(0, p.x, CTRL, 0xF, 0xF, false);
+    p.y = __builtin_amdgcn_update_dpp(0, p.y, CTRL, 0xF, 0xF, false);
+    return *reinterpret_cast<double *>(&p);
+}
+
+// Sum over each aligned group of G lanes, in every lane of the group: DPP
+// inside a 16-lane row (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror), a shuffle across rows.  Every lane adds the
+// same two partial sums, so all lanes hold the same bits.
+template <int G>
+__device__ __forceinline__ double group_sum_f64(double v)
+{
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32, "group size");
+    if constexpr (G >= 2) v += dpp_f64<0xB1>(v);
+    if constexpr (G >= 4) v += dpp_f64<0x4E>(v);
+    if constexpr (G >= 8) v += dpp_f64<0x141>(v);
+    if constexpr (G >= 16) v += dpp_f64<0x140>(v);
+    if constexpr (G >= 32) v += __shfl_xor(v, 16, 32);
+    return v;
+}
+
 // ---- closed-form polyharmonic spline derivatives ---------------------------
 // phi(v) = |v|^m (src/rbfbasis.jl:9).  d^a/dx^a d^b/dy^b of g^s, g = vx^2+vy^2,
 // s = m/2:

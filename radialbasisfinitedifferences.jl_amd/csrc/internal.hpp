@@ -132,4 +132,38 @@ struct CombineTerms {
 // out = sum_o coef[o] vals[o] (FP64, may be NULL) and its FP32 rounding (may be NULL).
 void launch_combine(long nnz, const CombineTerms &t, double *out, float *out32, hipStream_t s);
 
+// --------------------------------------------------- sparse solve (CGLS)
+// Grid-wide sum target: block sums go to partial[kLsqBlocks], the counter is
+// zero between launches, the total lands in *out.
+constexpr int kLsqBlocks = 1024;
+struct LsqReduce {
+    double *partial;
+    unsigned *count;
+    double *out;
+};
+// y = A x (CSR, int64 rowptr) and |y|^2 -> *red.out
+void launch_lsq_spmv(long M, long nnz, const int64_t *rowptr, const int *col, const double *val, const double *x,
+                     double *y, const LsqReduce &red, hipStream_t s);
+// z += a p, r -= a q with a = *gam / *qq
+void launch_lsq_update(long N, long M, const double *gam, const double *qq, const double *p, const double *q,
+                       double *z, double *r, hipStream_t s);
+// p = s + (*gam_new / *gam) p
+void launch_lsq_direction(long N, const double *gam, const double *gam_new, const double *sv, double *p,
+                          hipStream_t s);
+void launch_row_of(const int64_t *rowptr, long M, int *row_of, hipStream_t s);
+void launch_gather_i32(const int *src, const int *idx, long n, int *dst, hipStream_t s);
+void launch_col_scale(const int64_t *tptr, long N, const double *tval, double *scale, hipStream_t s);
+void launch_scale_cols(const int *col, long nnz, const double *scale, const double *v, double *out, hipStream_t s);
+void launch_copy_f64(const double *x, long n, double *y, hipStream_t s);
+void launch_lsq_finish(long N, const double *scale, const double *z, double *x, hipStream_t s);
+void launch_fill_f64(double *p, long n, double v, hipStream_t s);
+// Direct least squares: D (CSR) densified into A (M x N column-major
+// workspace), b copied to B (M), rocSOLVER QR solve, x = B[0:N].  *handle is
+// the caller's solver handle (created on first use).  Returns gels' info
+// (> 0: rank deficient).
+bool dense_solver_available();
+void dense_solver_destroy(void *handle);
+int dense_lsq(void **handle, long M, long N, const int64_t *rowptr, const int *col, const double *vals,
+              const double *b, double *x, double *A, double *B, int *info, hipStream_t s);
+
 }  // namespace rbffd

@@ -312,8 +312,49 @@ class Engine:
                                        order.ptr if order is not None else None))
         return y
 
+    def lsq_solve(self, rowptr, col, vals, b, x, method="auto", tol=1e-12, max_iters=100000):
+        """x = argmin ||D x - b|| for D (M x N, M >= N) in device CSR (int64 rowptr,
+        int32 columns, float64 values); x (device, N) holds the start (CGLS) and
+        gets the solution.  method: "auto", "qr" (direct) or "cgls".  Returns
+        (CGLS iterations, relative normal-equation residual)."""
+        M = b.shape[0]
+        N = x.shape[0]
+        it = _lib._I64(0)
+        rel = _lib._D(0.0)
+        m = {"auto": 0, "cgls": 1, "qr": 2}[method]
+        check(self.L.rbffd_lsq_solve_device(self.ctx.handle, M, N, rowptr.ptr, col.ptr, vals.ptr, b.ptr, x.ptr, m,
+                                            float(tol), int(max_iters), ctypes.byref(it), ctypes.byref(rel)))
+        return int(it.value), float(rel.value)
+
     def locality_order(self, X):
         """Morton order of the nodes of X (device, N x 2), as an int32 device array."""
         order = self.ctx.empty((X.shape[0],), np.int32)
         check(self.L.rbffd_locality_order_device(self.ctx.handle, X.ptr, X.shape[0], order.ptr))
         return order
+
+
+def solve(D, f, method="auto", tol=1e-12, max_iters=100000, ctx=None, info=False):
+    """`D \\ f` of the reference's PDE harness (test/poisson_test.jl:121,
+    test/mesh_import_test.jl:150) for a scipy sparse D with at least as many
+    rows as columns: the least-squares solution (the solution when square), on
+    the GPU by a dense QR ("qr") or CGLS on the column-equilibrated normal
+    equations ("cgls"); "auto" takes QR while the dense copy has at most 2^28
+    entries.  With info=True also returns (CGLS iterations, relative
+    normal-equation residual).  A rank-deficient D raises SingularException
+    (QR), as Julia's `\\` does for a singular square D."""
+    ctx = ctx or default_context()
+    A = sp.csr_matrix(D)
+    A.sort_indices()
+    M, N = A.shape
+    f = np.ascontiguousarray(np.asarray(f, dtype=np.float64).ravel())
+    if f.shape[0] != M:
+        raise ValueError("f must have one entry per row of D")
+    eng = Engine(ctx)
+    rowptr = ctx.to_device(A.indptr.astype(np.int64))
+    col = ctx.to_device(A.indices.astype(np.int32))
+    vals = ctx.to_device(A.data.astype(np.float64))
+    b = ctx.to_device(f)
+    x = ctx.zeros((N,), np.float64)
+    its, rel = eng.lsq_solve(rowptr, col, vals, b, x, method=method, tol=tol, max_iters=max_iters)
+    out = x.get()
+    return (out, (its, rel)) if info else out

@@ -251,8 +251,13 @@ def _csc(t):
     return sp.csc_matrix((nz, rv, cp), shape=sh)
 
 
-def poisson_error(O_or_ops, X, Y, yin, yneu, ydir, xn, yn, O):
-    E, Dx, Dy, Dxx, Dyy = O_or_ops
+def poisson_system(ops, X, Y, yin, yneu, ydir, xn, yn, O):
+    """The scaled collocation system of test/poisson_test.jl:74-113: Laplacian
+    rows at interior Y nodes, normal-derivative rows at Neumann nodes, E rows at
+    Dirichlet nodes, each block scaled by 1/sqrt(its row count) (Dirichlet also
+    by 1/h, h the mean nearest-neighbour distance of X).  Returns (A csr, b)
+    with the rows in block order."""
+    E, Dx, Dy, Dxx, Dyy = ops
     u = lambda x, y: np.sin(2 * np.pi * x * y)
     f2 = lambda x, y: x**2 * np.pi**2 * np.sin(x * y * np.pi * 2) * (-4) - y**2 * np.pi**2 * np.sin(x * y * np.pi * 2) * 4
     f1 = lambda n1, n2, x, y: n2 * x * np.pi * np.cos(x * y * np.pi * 2) * 2 + n1 * y * np.pi * np.cos(x * y * np.pi * 2) * 2
@@ -265,11 +270,20 @@ def poisson_error(O_or_ops, X, Y, yin, yneu, ydir, xn, yn, O):
     h = np.mean(np.sqrt(d2[:, 1]))
     M0, M1, M2 = len(ydir), len(yneu), len(yin)
     s = np.concatenate([np.full(M2, 1 / np.sqrt(M2)), np.full(M1, 1 / np.sqrt(M1)), np.full(M0, 1 / h / np.sqrt(M0))])
-    A = sp.diags(s) @ A
-    sol = np.linalg.lstsq(A.toarray(), s * f, rcond=None)[0]
+    return (sp.diags(s) @ A).tocsr(), s * f
+
+
+def poisson_rel_error(E, sol, Y):
+    """Relative L2 error of u_Y = E sol against sin(2 pi x y) (test/poisson_test.jl:125-127)."""
     uY = E @ sol
-    ue = u(Y[:, 0], Y[:, 1])
-    return np.linalg.norm(uY - ue) / np.linalg.norm(ue)
+    ue = np.sin(2 * np.pi * Y[:, 0] * Y[:, 1])
+    return float(np.linalg.norm(uY - ue) / np.linalg.norm(ue))
+
+
+def poisson_error(O_or_ops, X, Y, yin, yneu, ydir, xn, yn, O):
+    A, b = poisson_system(O_or_ops, X, Y, yin, yneu, ydir, xn, yn, O)
+    sol = np.linalg.lstsq(A.toarray(), b, rcond=None)[0]
+    return poisson_rel_error(O_or_ops[0], sol, Y)
 
 
 def load_poisson_case(O, fitted_nodes):
