@@ -9,7 +9,9 @@
 #                                                                src/hyperviscosity_operator.jl:26,177-179
 #   hyperviscosity_operator(k, X, p, n, polydeg)                 src/hyperviscosity_operator.jl:314
 #
-# Each returns SparseMatrixCSC{Float64,Int64} in the reference's layout:
+#   solve(D, f)  (= D \ f of test/poisson_test.jl:121)           least squares on the GPU
+#
+# Each generator returns SparseMatrixCSC{Float64,Int64} in the reference's layout:
 # 1-based, rows sorted inside each column, explicit zeros kept, dims
 # (number of Y nodes, largest referenced column).  Errors map to the
 # exceptions the reference raises (SingularException from inv, ArgumentError,
@@ -23,7 +25,7 @@ using LinearAlgebra
 using SparseArrays
 using StaticArrays
 
-export generate_operator, hyperviscosity_operator
+export generate_operator, hyperviscosity_operator, solve
 
 const LIB = get(ENV, "RBFFD_LIB", joinpath(@__DIR__, "..", "rbffd_amd", "librbffd.so"))
 
@@ -149,5 +151,50 @@ hyperviscosity_operator(k_deriv, X, Y, p, n, polydeg, X_idx_in, X_idx_bc, X_idx_
 
 hyperviscosity_operator(k_deriv, X, p, n, polydeg) =
     _operators(X, nothing, p, n, polydeg, Int32[OP_XK, OP_YK], k_deriv, nothing)
+
+# ---- D \ f of the PDE harness (test/poisson_test.jl:121), rbffd_lsq_solve_device
+const SOLVE_METHODS = Dict(:auto => Int32(0), :cgls => Int32(1), :qr => Int32(2))
+
+function _dalloc(c, bytes)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:rbffd_device_malloc, LIB), Cint, (Ptr{Cvoid}, Int64, Ref{Ptr{Cvoid}}), c, max(bytes, 16), r))
+    return r[]
+end
+
+_dfree(c, p) = ccall((:rbffd_device_free, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), c, p)
+
+function _upload(c, a::Vector)
+    p = _dalloc(c, sizeof(a))
+    GC.@preserve a check(ccall((:rbffd_memcpy_h2d, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                               c, p, pointer(a), sizeof(a)))
+    return p
+end
+
+function solve(D::SparseMatrixCSC{Float64}, f::AbstractVector{Float64}; method::Symbol = :auto,
+               tol::Float64 = 1e-12, max_iters::Integer = 100_000)
+    M, N = size(D)
+    Dt = sparse(transpose(D))                  # the CSC of D' is the CSR of D
+    c = context()
+    bufs = Ptr{Cvoid}[]
+    try
+        rowptr = _upload(c, Int64.(Dt.colptr .- 1)); push!(bufs, rowptr)
+        col = _upload(c, Int32.(Dt.rowval .- 1)); push!(bufs, col)
+        vals = _upload(c, Vector{Float64}(Dt.nzval)); push!(bufs, vals)
+        b = _upload(c, Vector{Float64}(f)); push!(bufs, b)
+        x = _upload(c, zeros(N)); push!(bufs, x)
+        iters = Ref{Int64}(0)
+        relres = Ref{Float64}(0.0)
+        check(ccall((:rbffd_lsq_solve_device, LIB), Cint,
+                    (Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                     Int32, Float64, Int64, Ref{Int64}, Ref{Float64}),
+                    c, M, N, rowptr, col, vals, b, x, SOLVE_METHODS[method], tol, max_iters, iters, relres))
+        out = Vector{Float64}(undef, N)
+        GC.@preserve out check(ccall((:rbffd_memcpy_d2h, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                                     c, pointer(out), x, sizeof(out)))
+        return out
+    finally
+        foreach(p -> _dfree(c, p), bufs)
+    end
+end
 
 end # module
