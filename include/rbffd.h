@@ -244,6 +244,40 @@ int rbffd_ssprk3_device(rbffd_context *ctx, int64_t N, const int64_t *rowptr, co
                         int32_t vals_type, int32_t uniform_n, const int32_t *row_order, int32_t F, double *u,
                         double *work, const uint8_t *fixed, double dt, int64_t steps);
 
+/* Ghost-node closure of one boundary marker (examples/adv_diff_test.jl:107-146,
+ * 168-182): the marker's ghost values are set so that its boundary rows of the
+ * normal-derivative operator D (Dx on left/right markers, Dy on top/bottom)
+ * vanish,
+ *     u[ghost] = -inv(D[bc, ghost]) * (D[bc, not ghost] * u).
+ * All pointers are device pointers. */
+typedef struct rbffd_ghost_closure {
+    int64_t nbc;            /* boundary nodes of the marker = its ghost nodes */
+    const int64_t *rowptr;  /* nbc + 1: CSR of D[bc, :] without this marker's ghost columns (w_bc_int) */
+    const int32_t *col;     /* global node indices */
+    const double *vals;
+    const double *ginv;     /* nbc x nbc column-major inv(D[bc, ghost]) (w_bc_inv) */
+    const int32_t *ghost;   /* the marker's ghost nodes, in the column order of D[bc, ghost] */
+} rbffd_ghost_closure;
+
+/* rbffd_ssprk3_device with ghost-node closures: the closures run in the
+ * given order on u before the first step and on the output of every stage,
+ * so every stage reads consistent ghost values (the reference refreshes them
+ * inside each right-hand-side evaluation, examples/adv_diff_test.jl:168-182).
+ * closures: host array of nclosures entries.  Sums run in a fixed order:
+ * results are bitwise reproducible.  Asynchronous after the argument checks
+ * (which read back a few flags). */
+int rbffd_ssprk3_closure_device(rbffd_context *ctx, int64_t N, const int64_t *rowptr, const int32_t *col,
+                                const void *vals, int32_t vals_type, int32_t uniform_n, const int32_t *row_order,
+                                int32_t F, double *u, double *work, const uint8_t *fixed, int32_t nclosures,
+                                const rbffd_ghost_closure *closures, double dt, int64_t steps);
+
+/* Ainv = inv(A) for a dense n x n column-major device matrix, by rocSOLVER's
+ * QR solve of A X = I (librocsolver.so.0 opened at run time;
+ * RBFFD_ERR_UNSUPPORTED without it).  Replaces `inv(Array(w_bc_g[m]))` of the
+ * ghost closure (examples/adv_diff_test.jl:118,124,130,136).  A singular A
+ * gives RBFFD_ERR_SINGULAR.  Needs n^2 < 2^31.  Synchronises. */
+int rbffd_dense_inverse_device(rbffd_context *ctx, int64_t n, const double *A, double *Ainv);
+
 /* ------------------------------------------------------ uneven batch
  * Many independent node sets, each with its own stencil size, polynomial
  * degree and PHS power, generated in one call (Y = X for every set; method 1,

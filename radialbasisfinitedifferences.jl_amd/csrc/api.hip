@@ -100,7 +100,8 @@ enum Slot {
     S_BBOX_PART, S_BBOX, S_FLAG, S_XGROUP, S_QGROUP, S_STENCILS, S_Y2X, S_STATUS, S_ROWS, S_CSC_COUNTS,
     S_CSC_PTR, S_CSC_FILL, S_CSC_PERM, S_CSC_ROW, S_H_X, S_H_Y, S_H_COL, S_H_VALS, S_H_CVALS,
     S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS, S_MORTON_KEY, S_MORTON_IDX, S_CSC_PERM_IN, S_LSQ_ROWOF, S_LSQ_TCOL, S_LSQ_TPTR, S_LSQ_F64,
-    S_LSQ_COUNT, S_LSQ_DENSE, S_LSQ_INFO, S_NSLOT
+    S_LSQ_COUNT, S_LSQ_DENSE, S_LSQ_INFO, S_CLOSURE_T,
+    S_CLOSURE_PART, S_NSLOT
 };
 
 struct rbffd_context {
@@ -621,6 +622,47 @@ __global__ void count_decreasing_kernel(const int64_t *rowptr, long M, int *flag
     if (i < M && rowptr[i + 1] < rowptr[i]) atomicAdd(flag, 1);
 }
 
+// Checks a CSR (int64 rowptr, int32 columns) on the device: rowptr starts at
+// 0 and never decreases, every column is in [0, ncols), nnz < 2^31.  Returns nnz.
+long validate_csr(rbffd_context *c, long M, const int64_t *rowptr, const int *col, long ncols, const char *what)
+{
+    hipStream_t st = c->stream;
+    int64_t ends[2] = {0, 0};
+    RBFFD_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RBFFD_HIP(hipMemcpyAsync(&ends[1], rowptr + M, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RBFFD_HIP(hipStreamSynchronize(st));
+    const long nnz = (long)ends[1];
+    if (ends[0] != 0 || nnz < 0) throw Error{RBFFD_ERR_ARG, std::string(what) + ": rowptr must start at 0 and end at nnz"};
+    if (nnz > INT_MAX) throw Error{RBFFD_ERR_ARG, std::string(what) + ": more than 2^31 - 1 entries"};
+    if (nnz == 0) return 0;
+    int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
+    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 3, st));
+    max_kernel<<<(unsigned)std::min<long>(2048, nblk(nnz)), 256, 0, st>>>(col, nnz, flag);
+    count_negative_kernel<<<nblk(nnz), 256, 0, st>>>(col, nnz, flag + 1);
+    count_decreasing_kernel<<<nblk(M), 256, 0, st>>>(rowptr, M, flag + 2);
+    int hf[3] = {0, 0, 0};
+    RBFFD_HIP(hipMemcpyAsync(hf, flag, sizeof(hf), hipMemcpyDeviceToHost, st));
+    RBFFD_HIP(hipStreamSynchronize(st));
+    if (hf[1] > 0 || hf[0] >= ncols) throw Error{RBFFD_ERR_ARG, std::string(what) + ": column index out of range"};
+    if (hf[2] > 0) throw Error{RBFFD_ERR_ARG, std::string(what) + ": rowptr decreases"};
+    return nnz;
+}
+
+// Indices all in [0, n)
+void validate_indices(rbffd_context *c, const int *idx, long count, long n, const char *what)
+{
+    if (count <= 0) return;
+    hipStream_t st = c->stream;
+    int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
+    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 2, st));
+    max_kernel<<<(unsigned)std::min<long>(2048, nblk(count)), 256, 0, st>>>(idx, count, flag);
+    count_negative_kernel<<<nblk(count), 256, 0, st>>>(idx, count, flag + 1);
+    int hf[2] = {0, 0};
+    RBFFD_HIP(hipMemcpyAsync(hf, flag, sizeof(hf), hipMemcpyDeviceToHost, st));
+    RBFFD_HIP(hipStreamSynchronize(st));
+    if (hf[1] > 0 || hf[0] >= n) throw Error{RBFFD_ERR_ARG, std::string(what) + ": index out of range"};
+}
+
 // Iterations per recorded graph.  Even: the two gamma slots alternate by
 // iteration parity, and every replay starts at an even iteration.
 constexpr long kLsqChunk = 32;
@@ -664,24 +706,8 @@ void lsq_impl(rbffd_context *c, long M, long N, const int64_t *rowptr, const int
     if (M > INT_MAX || N > INT_MAX) throw Error{RBFFD_ERR_ARG, "lsq: more than 2^31 - 1 rows"};
     if (!(tol >= 0.0) || max_iters < 0) throw Error{RBFFD_ERR_ARG, "lsq: tol and max_iters must be >= 0"};
     if (N == 0) return;
-    int64_t ends[2] = {0, 0};
-    RBFFD_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    RBFFD_HIP(hipMemcpyAsync(&ends[1], rowptr + M, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    RBFFD_HIP(hipStreamSynchronize(st));
-    const long nnz = (long)ends[1];
-    if (ends[0] != 0 || nnz < 0) throw Error{RBFFD_ERR_ARG, "lsq: rowptr must start at 0 and end at nnz"};
-    if (nnz > INT_MAX) throw Error{RBFFD_ERR_ARG, "lsq: more than 2^31 - 1 entries"};
+    const long nnz = validate_csr(c, M, rowptr, col, N, "lsq");
     if (nnz == 0) return;  // D = 0: every x is a least-squares solution; x stays the start
-    int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
-    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 3, st));
-    max_kernel<<<(unsigned)std::min<long>(2048, nblk(nnz)), 256, 0, st>>>(col, nnz, flag);
-    count_negative_kernel<<<nblk(nnz), 256, 0, st>>>(col, nnz, flag + 1);
-    count_decreasing_kernel<<<nblk(M), 256, 0, st>>>(rowptr, M, flag + 2);
-    int hf[3] = {0, 0, 0};
-    RBFFD_HIP(hipMemcpyAsync(hf, flag, sizeof(hf), hipMemcpyDeviceToHost, st));
-    RBFFD_HIP(hipStreamSynchronize(st));
-    if (hf[1] > 0 || hf[0] >= N) throw Error{RBFFD_ERR_ARG, "lsq: column index out of range"};
-    if (hf[2] > 0) throw Error{RBFFD_ERR_ARG, "lsq: rowptr decreases"};
 
     if (method == RBFFD_SOLVE_AUTO)
         method = (rbffd::dense_solver_available() && M * N <= kDenseAutoEntries) ? RBFFD_SOLVE_QR : RBFFD_SOLVE_CGLS;
@@ -1293,6 +1319,74 @@ int rbffd_ssprk3_device(rbffd_context *c, int64_t N, const int64_t *rowptr, cons
                                    0.25 * dt, s);
             rbffd::launch_rk_stage(N, rowptr, col, vals, f32, F, uniform_n, o, u2, u, u, fixed, 1.0 / 3.0,
                                    2.0 / 3.0, 2.0 / 3.0 * dt, s);
+        }
+        RBFFD_HIP(hipGetLastError());
+    });
+}
+
+int rbffd_dense_inverse_device(rbffd_context *c, int64_t n, const double *A, double *Ainv)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (n < 0 || (n > 0 && (!A || !Ainv))) throw Error{RBFFD_ERR_ARG, "inverse: bad arguments"};
+        if (n == 0) return;
+        if (n * n > (int64_t)INT_MAX) throw Error{RBFFD_ERR_ARG, "inverse: needs n^2 < 2^31"};
+        double *Aw = (double *)ws(c, S_LSQ_DENSE, sizeof(double) * (size_t)n * n);
+        int *info = (int *)ws(c, S_LSQ_INFO, sizeof(int) * 4);
+        RBFFD_HIP(hipMemcpyAsync(Aw, A, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, c->stream));
+        const int r = rbffd::dense_inverse(&c->solver, n, Aw, Ainv, info, c->stream);
+        if (r > 0) throw Error{RBFFD_ERR_SINGULAR, "inverse: the matrix is singular"};
+    });
+}
+
+int rbffd_ssprk3_closure_device(rbffd_context *c, int64_t N, const int64_t *rowptr, const int32_t *col,
+                                const void *vals, int32_t vals_type, int32_t uniform_n, const int32_t *row_order,
+                                int32_t F, double *u, double *work, const uint8_t *fixed, int32_t nclosures,
+                                const rbffd_ghost_closure *closures, double dt, int64_t steps)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (N < 0 || !col || !vals || !u || !work || steps < 0) throw Error{RBFFD_ERR_ARG, "ssprk3: bad arguments"};
+        if (uniform_n <= 0 && !rowptr) throw Error{RBFFD_ERR_ARG, "ssprk3: rowptr or uniform_n needed"};
+        if (vals_type != RBFFD_F64 && vals_type != RBFFD_F32) throw Error{RBFFD_ERR_ARG, "ssprk3: value type"};
+        if (F != 1 && F != 2 && F != 4 && F != 8) throw Error{RBFFD_ERR_UNSUPPORTED, "ssprk3: fields must be 1, 2, 4 or 8"};
+        if (nclosures < 0 || (nclosures > 0 && !closures)) throw Error{RBFFD_ERR_ARG, "ssprk3: bad closures"};
+        long maxb = 0;
+        for (int m = 0; m < nclosures; ++m) {
+            const rbffd_ghost_closure &g = closures[m];
+            if (g.nbc < 1 || !g.rowptr || !g.col || !g.vals || !g.ginv || !g.ghost)
+                throw Error{RBFFD_ERR_ARG, "ssprk3: closure " + std::to_string(m) + " is incomplete"};
+            if (g.nbc > INT_MAX / 8) throw Error{RBFFD_ERR_ARG, "ssprk3: closure too large"};
+            validate_csr(c, g.nbc, g.rowptr, g.col, N, "ssprk3 closure");
+            validate_indices(c, g.ghost, g.nbc, N, "ssprk3 closure ghosts");
+            maxb = std::max<long>(maxb, g.nbc);
+        }
+        hipStream_t s = c->stream;
+        double *t = (double *)ws(c, S_CLOSURE_T, sizeof(double) * (size_t)std::max<long>(maxb, 1) * F);
+        double *part = (double *)ws(c, S_CLOSURE_PART,
+                                    sizeof(double) * (size_t)std::max<long>(rbffd::closure_chunks(maxb), 1) *
+                                        std::max<long>(maxb, 1) * F);
+        // ghost values of every marker, in marker order, from the current v
+        auto close = [&](double *v) {
+            for (int m = 0; m < nclosures; ++m) {
+                const rbffd_ghost_closure &g = closures[m];
+                rbffd::launch_spmv(g.nbc, g.rowptr, g.col, g.vals, 0, F, v, t, 1.0, 0.0, 0, nullptr, s);
+                rbffd::launch_closure(g.nbc, F, g.ginv, t, part, g.ghost, v, s);
+            }
+        };
+        const int f32 = vals_type == RBFFD_F32;
+        double *u1 = work, *u2 = work + (size_t)N * F;
+        const int *o = row_order;
+        close(u);
+        for (int64_t k = 0; k < steps; ++k) {
+            rbffd::launch_rk_stage(N, rowptr, col, vals, f32, F, uniform_n, o, u, u, u1, fixed, 0.0, 1.0, dt, s);
+            close(u1);
+            rbffd::launch_rk_stage(N, rowptr, col, vals, f32, F, uniform_n, o, u1, u, u2, fixed, 0.75, 0.25,
+                                   0.25 * dt, s);
+            close(u2);
+            rbffd::launch_rk_stage(N, rowptr, col, vals, f32, F, uniform_n, o, u2, u, u, fixed, 1.0 / 3.0,
+                                   2.0 / 3.0, 2.0 / 3.0 * dt, s);
+            close(u);
         }
         RBFFD_HIP(hipGetLastError());
     });

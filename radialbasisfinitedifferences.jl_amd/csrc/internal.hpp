@@ -118,6 +118,11 @@ void launch_f64_to_f32(const double *src, long n, float *dst, hipStream_t s);
 void launch_rk_stage(long M, const int64_t *rowptr, const int *col, const void *vals, int vals_f32, int F,
                      int uniform_n, const int *order, const double *w, const double *u, double *out,
                      const uint8_t *fixed, double a, double b, double c, hipStream_t s);
+// Ghost-node closure: u[ghost[i]] = -(G t)_i for i < nbc (F fields), G dense
+// column-major nbc x nbc, t = nbc x F; part holds closure_chunks(nbc) x nbc x F.
+long closure_chunks(long nbc);
+void launch_closure(long nbc, int F, const double *G, const double *t, double *part, const int *ghost, double *u,
+                    hipStream_t s);
 // Rows of N nodes in Morton order of their coordinates (box from launch_bbox);
 // key, key_sorted, idx: N-entry workspaces; tmp of morton_tmp_bytes(N).
 size_t morton_tmp_bytes(long N);
@@ -165,5 +170,8 @@ bool dense_solver_available();
 void dense_solver_destroy(void *handle);
 int dense_lsq(void **handle, long M, long N, const int64_t *rowptr, const int *col, const double *vals,
               const double *b, double *x, double *A, double *B, int *info, hipStream_t s);
+// Ainv = inv(A), n x n column-major, through rocSOLVER's QR solve of A X = I;
+// Aw (n x n) is overwritten.  Returns gels' info (> 0: singular).
+int dense_inverse(void **handle, long n, double *Aw, double *Ainv, int *info, hipStream_t s);
 
 }  // namespace rbffd

@@ -319,4 +319,33 @@ int dense_lsq(void **handle, long M, long N, const int64_t *rowptr, const int *c
     return hinfo;
 }
 
+// A X = I by QR: X = inv(A)
+__global__ void identity_kernel(long n, double *X)
+{
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e < n * n) X[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
+}
+
+int dense_inverse(void **handle, long n, double *Aw, double *Ainv, int *info, hipStream_t s)
+{
+    const SolverLib *L = solver_lib();
+    if (!L) throw Error{2, "inverse: needs rocSOLVER (librocsolver.so.0), which did not load"};
+    if (!*handle) {
+        rocblas_handle h = nullptr;
+        if (L->create(&h) != rocblas_status_success) throw Error{3, "inverse: rocblas_create_handle failed"};
+        *handle = h;
+    }
+    rocblas_handle h = (rocblas_handle)*handle;
+    if (L->set_stream(h, s) != rocblas_status_success) throw Error{3, "inverse: rocblas_set_stream failed"};
+    identity_kernel<<<(unsigned)((n * n + 255) / 256), 256, 0, s>>>(n, Ainv);
+    RBFFD_HIP(hipGetLastError());
+    const rocblas_status st = L->dgels(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, (rocblas_int)n, Aw,
+                                       (rocblas_int)n, Ainv, (rocblas_int)n, info);
+    if (st != rocblas_status_success) throw Error{3, "inverse: rocsolver_dgels failed (status " + std::to_string((int)st) + ")"};
+    int hinfo = 0;
+    RBFFD_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
+    RBFFD_HIP(hipStreamSynchronize(s));
+    return hinfo;
+}
+
 }  // namespace rbffd

@@ -379,4 +379,64 @@ void launch_combine(long nnz, const CombineTerms &t, double *out, float *out32, 
     combine_kernel<<<(unsigned)((nnz + 255) / 256), 256, 0, s>>>(nnz, t, out, out32);
 }
 
+// ---- ghost-node closure (examples/adv_diff_test.jl:168-182):
+// u[ghost] = -G t with G = inv(D[bc, ghost]) dense column-major (nbc x nbc)
+// and t = D[bc, not ghost] u (nbc x F).  The product runs in chunks of
+// kClosureChunk columns of G; the chunk sums are added in chunk order, so the
+// result does not depend on scheduling.
+constexpr int kClosureChunk = 256;
+
+long closure_chunks(long nbc) { return (nbc + kClosureChunk - 1) / kClosureChunk; }
+
+template <int F>
+__global__ __launch_bounds__(256) void closure_gemv_kernel(long nbc, const double *__restrict__ G,
+                                                           const double *__restrict__ t, double *__restrict__ part)
+{
+    __shared__ double ts[kClosureChunk * F];
+    const long j0 = (long)blockIdx.y * kClosureChunk;
+    const int nj = (int)((nbc - j0) < kClosureChunk ? (nbc - j0) : kClosureChunk);
+    for (int e = threadIdx.x; e < nj * F; e += blockDim.x) ts[e] = t[j0 * F + e];
+    __syncthreads();
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= nbc) return;
+    double acc[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc[f] = 0.0;
+    for (int j = 0; j < nj; ++j) {
+        const double g = G[(j0 + j) * nbc + i];
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc[f] = __builtin_fma(g, ts[j * F + f], acc[f]);
+    }
+    double *o = part + ((long)blockIdx.y * nbc + i) * F;
+#pragma unroll
+    for (int f = 0; f < F; ++f) o[f] = acc[f];
+}
+
+__global__ void closure_scatter_kernel(long nbc, int F, long nchunks, const double *__restrict__ part,
+                                       const int *__restrict__ ghost, double *__restrict__ u)
+{
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= nbc * F) return;
+    double s = 0.0;
+    for (long c = 0; c < nchunks; ++c) s += part[c * nbc * F + e];
+    const long i = e / F;
+    u[(long)ghost[i] * F + (e - i * F)] = -s;
+}
+
+void launch_closure(long nbc, int F, const double *G, const double *t, double *part, const int *ghost, double *u,
+                    hipStream_t s)
+{
+    if (nbc <= 0) return;
+    const long nc = closure_chunks(nbc);
+    dim3 grid((unsigned)((nbc + 255) / 256), (unsigned)nc);
+    switch (F) {
+    case 1: closure_gemv_kernel<1><<<grid, 256, 0, s>>>(nbc, G, t, part); break;
+    case 2: closure_gemv_kernel<2><<<grid, 256, 0, s>>>(nbc, G, t, part); break;
+    case 4: closure_gemv_kernel<4><<<grid, 256, 0, s>>>(nbc, G, t, part); break;
+    case 8: closure_gemv_kernel<8><<<grid, 256, 0, s>>>(nbc, G, t, part); break;
+    default: throw Error{2, "closure: fields must be 1, 2, 4 or 8"};
+    }
+    closure_scatter_kernel<<<(unsigned)((nbc * F + 255) / 256), 256, 0, s>>>(nbc, F, nc, part, ghost, u);
+}
+
 }  // namespace rbffd

@@ -70,6 +70,11 @@ class rbffd_stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class rbffd_ghost_closure(ctypes.Structure):
+    _fields_ = [("nbc", ctypes.c_int64), ("rowptr", ctypes.c_void_p), ("col", ctypes.c_void_p),
+                ("vals", ctypes.c_void_p), ("ginv", ctypes.c_void_p), ("ghost", ctypes.c_void_p)]
+
+
 class rbffd_batch_set(ctypes.Structure):
     _fields_ = [
         ("X", ctypes.c_void_p),
@@ -121,6 +126,9 @@ SIGNATURES = {
                                               ctypes.POINTER(_I64), ctypes.POINTER(_D)]),
     "rbffd_combine_device": (ctypes.c_int, [_V, _I64, _I32, _V, _V, _V, _V]),
     "rbffd_ssprk3_device": (ctypes.c_int, [_V, _I64, _V, _V, _V, _I32, _I32, _V, _I32, _V, _V, _V, _D, _I64]),
+    "rbffd_ssprk3_closure_device": (ctypes.c_int, [_V, _I64, _V, _V, _V, _I32, _I32, _V, _I32, _V, _V, _V, _I32,
+                                                   ctypes.POINTER(rbffd_ghost_closure), _D, _I64]),
+    "rbffd_dense_inverse_device": (ctypes.c_int, [_V, _I64, _V, _V]),
     "rbffd_boundary_groups": (ctypes.c_int, [ctypes.POINTER(rbffd_boundary), _I64, _V, _V]),
     "rbffd_generate_batch_device": (ctypes.c_int, [_V, _I32, ctypes.POINTER(rbffd_batch_set), _I32, _V, _V, _I32,
                                                    _V, ctypes.POINTER(rbffd_stats)]),

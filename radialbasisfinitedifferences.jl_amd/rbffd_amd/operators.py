@@ -28,7 +28,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib
-from ._lib import OP, PREC, check, rbffd_batch_set, rbffd_boundary, rbffd_stats
+from ._lib import OP, PREC, check, rbffd_batch_set, rbffd_boundary, rbffd_ghost_closure, rbffd_stats
 from .device import Context, DeviceArray
 
 _default_ctx = None
@@ -290,6 +290,34 @@ class Engine:
                                          fixed.ptr if fixed is not None else None, float(dt), int(steps)))
         return u
 
+    def ssprk3_closure(self, col, vals, u, dt, steps, closures, uniform_n=0, rowptr=None, fixed=None, work=None,
+                       order=None):
+        """ssprk3 with ghost-node closures: `closures` is a list of GhostClosure
+        (device arrays), applied in order before the first step and after every
+        stage."""
+        N = u.shape[0]
+        F = 1 if len(u.shape) == 1 else u.shape[1]
+        if work is None:
+            work = self.ctx.empty((2 * N * F,), np.float64)
+        vt = 1 if vals.dtype == np.float32 else 0
+        arr = (rbffd_ghost_closure * max(1, len(closures)))()
+        for i, g in enumerate(closures):
+            arr[i] = rbffd_ghost_closure(g.nbc, g.rowptr.ptr, g.col.ptr, g.vals.ptr, g.ginv.ptr, g.ghost.ptr)
+        check(self.L.rbffd_ssprk3_closure_device(self.ctx.handle, N, rowptr.ptr if rowptr is not None else None,
+                                                 col.ptr, vals.ptr, vt, int(uniform_n),
+                                                 order.ptr if order is not None else None, F, u.ptr, work.ptr,
+                                                 fixed.ptr if fixed is not None else None, len(closures), arr,
+                                                 float(dt), int(steps)))
+        return u
+
+    def dense_inverse(self, A):
+        """inv(A) of a dense n x n device matrix (column-major, i.e. the
+        transpose of a C-ordered array) as a new device array."""
+        n = A.shape[0]
+        out = self.ctx.empty((n, n), np.float64)
+        check(self.L.rbffd_dense_inverse_device(self.ctx.handle, n, A.ptr, out.ptr))
+        return out
+
     def csr_to_csc(self, colidx, vals, ncols, index_base=0):
         M, n = colidx.shape
         colptr = self.ctx.empty((ncols + 1,), np.int64)
@@ -358,3 +386,32 @@ def solve(D, f, method="auto", tol=1e-12, max_iters=100000, ctx=None, info=False
     its, rel = eng.lsq_solve(rowptr, col, vals, b, x, method=method, tol=tol, max_iters=max_iters)
     out = x.get()
     return (out, (its, rel)) if info else out
+
+
+class GhostClosure:
+    """Device data of one marker's ghost-node closure
+    (examples/adv_diff_test.jl:107-146): the CSR of D[bc, :] without the
+    marker's ghost columns, inv(D[bc, ghost]) and the ghost node indices."""
+
+    def __init__(self, ctx, D, bc, ghost, eng=None):
+        """D: scipy sparse N x N normal-derivative operator of the marker (Dx or
+        Dy); bc, ghost: its boundary nodes and their ghost nodes, paired in
+        order.  inv(D[bc, ghost]) is computed on the device."""
+        D = sp.csr_matrix(D)
+        bc = np.asarray(bc, dtype=np.int64)
+        ghost = np.asarray(ghost, dtype=np.int64)
+        rows = D[bc]
+        keep = np.ones(D.shape[1], dtype=bool)
+        keep[ghost] = False
+        inner = (rows @ sp.diags(keep.astype(np.float64))).tocsr()
+        inner.eliminate_zeros()
+        inner.sort_indices()
+        Wg = rows[:, ghost].toarray()
+        eng = eng or Engine(ctx)
+        self.nbc = len(bc)
+        self.rowptr = ctx.to_device(inner.indptr.astype(np.int64))
+        self.col = ctx.to_device(inner.indices.astype(np.int32))
+        self.vals = ctx.to_device(inner.data.astype(np.float64))
+        self.ginv = eng.dense_inverse(ctx.to_device(np.ascontiguousarray(Wg.T)))  # column-major
+        self.ghost = ctx.to_device(ghost.astype(np.int32))
+        self.host = (inner, Wg, bc, ghost)
