@@ -58,6 +58,8 @@ extern "C" {
 #define RBFFD_PREC_FP64 0   /* FP64 LU with partial pivoting */
 #define RBFFD_PREC_MIXED 1  /* FP32 Gauss-Jordan inverse + FP64 refinement, FP64 re-solve where it
                                does not converge */
+#define RBFFD_PREC_AUTO 2   /* the faster of the two on this device: FP64 on MI355X, where FP32
+                               issues at the FP64 rate without packed FP32 */
 
 /* value types for rbffd_spmv_device */
 #define RBFFD_F64 0

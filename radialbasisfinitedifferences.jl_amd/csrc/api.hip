@@ -384,6 +384,11 @@ int choose_group(rbffd_context *c, const rbffd::SolveArgs &a, bool may_tune = tr
     return pick;
 }
 
+// The precision RBFFD_PREC_AUTO resolves to.  FP64: on MI355X the FP32 bulk
+// solve issues at the FP64 rate (packed FP32 is not used), so the mixed path
+// only adds refinement work.
+int auto_precision() { return RBFFD_PREC_FP64; }
+
 void check_weights_args(int n, int p, int polydeg, int method, int nops, const int *ops, const int *opk,
                         int precision)
 {
@@ -391,8 +396,9 @@ void check_weights_args(int n, int p, int polydeg, int method, int nops, const i
     if (p < 1) throw Error{RBFFD_ERR_ARG, "PHS power must be >= 1"};
     if (polydeg < 0 || polydeg > 8) throw Error{RBFFD_ERR_ARG, "polynomial degree must be in 0..8"};
     check_ops(nops, ops, opk);
-    if (precision != RBFFD_PREC_FP64 && precision != RBFFD_PREC_MIXED)
+    if (precision != RBFFD_PREC_FP64 && precision != RBFFD_PREC_MIXED && precision != RBFFD_PREC_AUTO)
         throw Error{RBFFD_ERR_ARG, "unknown precision"};
+    if (precision == RBFFD_PREC_AUTO) precision = auto_precision();
     const int P = n_monomials(polydeg);
     if (!rbffd::solve_supported(n, P, precision))
         throw Error{RBFFD_ERR_UNSUPPORTED, "no compiled kernel for n=" + std::to_string(n) +
@@ -446,6 +452,7 @@ void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, lo
                   double *const *vals, int *status)
 {
     (void)N;
+    if (precision == RBFFD_PREC_AUTO) precision = auto_precision();
     check_weights_args(n, p, polydeg, method, nops, ops, opk, precision);
     solves_enqueue(c, X, Y, M, stencils, n, y2x, rows, nrows, p, polydeg, method, nops, ops, opk, precision,
                    colidx, vals, status, c->stream, true);
@@ -489,6 +496,7 @@ void generate_impl(rbffd_context *c, const double *X, long N, const double *Y, l
                    rbffd_stats *stats, int *status_codes)
 {
     hipStream_t s = c->stream;
+    if (precision == RBFFD_PREC_AUTO) precision = auto_precision();
     const bool m3 = (Y == nullptr);
     if (m3) { Y = X; M = N; }
     if (n < 1 || n > N) throw Error{RBFFD_ERR_ARG, "stencil size must be in 1..number of X nodes"};

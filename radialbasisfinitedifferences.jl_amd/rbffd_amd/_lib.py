@@ -19,7 +19,7 @@ RBFFD_ERR_NONFINITE = 5
 RBFFD_ERR_UNDEFINED = 6
 
 OP = {"E": 0, "Dx": 1, "Dy": 2, "Dxx": 3, "Dyy": 4, "Dxy": 5, "Dxk": 6, "Dyk": 7, "Lap": 8, "Lapk": 9}
-PREC = {"fp64": 0, "mixed": 1}
+PREC = {"fp64": 0, "mixed": 1, "auto": 2}
 
 
 class SingularException(ArithmeticError):

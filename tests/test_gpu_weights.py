@@ -135,6 +135,10 @@ def test_generate_pipeline_mixed_falls_back(ctx, O):
     c2, v2, s2 = eng.generate(Xd, Xd, p, n, polydeg, ops, precision="mixed")
     print("  stats", s1, s2, flush=True)
     assert np.array_equal(c1.get(), c2.get())
+    # "auto" resolves to FP64 on this device: the same bits
+    c3, v3, _ = eng.generate(Xd, Xd, p, n, polydeg, ops, precision="auto")
+    assert np.array_equal(c3.get(), c1.get())
+    assert all(np.array_equal(a.get(), b.get()) for a, b in zip(v1, v3))
     st, _ = O.knn(X, X, n)
     Wq, _, _ = O.weights(X, X, st, None, p, polydeg, _opl(O, ops, 0), quad=True)
     for o in range(3):
