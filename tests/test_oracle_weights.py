@@ -258,13 +258,16 @@ def poisson_system(ops, X, Y, yin, yneu, ydir, xn, yn, O):
     by 1/h, h the mean nearest-neighbour distance of X).  Returns (A csr, b)
     with the rows in block order."""
     E, Dx, Dy, Dxx, Dyy = ops
+    # normals: a column of the reference's CSV fixtures, or plain vectors
+    xn = xn if xn.ndim == 1 else xn[:, 2]
+    yn = yn if yn.ndim == 1 else yn[:, 2]
     u = lambda x, y: np.sin(2 * np.pi * x * y)
     f2 = lambda x, y: x**2 * np.pi**2 * np.sin(x * y * np.pi * 2) * (-4) - y**2 * np.pi**2 * np.sin(x * y * np.pi * 2) * 4
     f1 = lambda n1, n2, x, y: n2 * x * np.pi * np.cos(x * y * np.pi * 2) * 2 + n1 * y * np.pi * np.cos(x * y * np.pi * 2) * 2
     L = (Dxx + Dyy).tocsr()
-    A = sp.vstack([L[yin], sp.diags(xn[:, 2]) @ Dx.tocsr()[yneu] + sp.diags(yn[:, 2]) @ Dy.tocsr()[yneu],
+    A = sp.vstack([L[yin], sp.diags(xn) @ Dx.tocsr()[yneu] + sp.diags(yn) @ Dy.tocsr()[yneu],
                    E.tocsr()[ydir]]).tocsr()
-    f = np.concatenate([f2(Y[yin, 0], Y[yin, 1]), f1(xn[:, 2], yn[:, 2], Y[yneu, 0], Y[yneu, 1]),
+    f = np.concatenate([f2(Y[yin, 0], Y[yin, 1]), f1(xn, yn, Y[yneu, 0], Y[yneu, 1]),
                         u(Y[ydir, 0], Y[ydir, 1])])
     _, d2 = O.knn(X, X, 2)
     h = np.mean(np.sqrt(d2[:, 1]))
