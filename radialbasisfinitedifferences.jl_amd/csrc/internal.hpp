@@ -51,10 +51,12 @@ void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tm
 // Scatter points into cell order (fill must be zeroed).
 void launch_cell_scatter(const double *X, const int *xgroup, long N, const int *cell_of, const int *start,
                          int *fill, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s);
-// Exact kNN of every query, ascending (d^2, index).
+// Exact kNN of every query, ascending (d^2, index).  redo (M ints, may be
+// NULL) and redo_count (one int) are the workspace of the two-phase selection
+// used for k > 8; without them every query runs the insertion kernel.
 void launch_knn_query(const GridParams *g, const int *cell_start, const double *cell_xy, const int *cell_idx,
                       const int *cell_grp, const double *Q, const int *qgroup, const int *qorder, long M,
-                      int k, int *idx_out, double *d2_out, hipStream_t s);
+                      int k, int *idx_out, double *d2_out, int *redo, int *redo_count, hipStream_t s);
 
 // ------------------------------------------------------- local solves (K2)
 struct SolveArgs {

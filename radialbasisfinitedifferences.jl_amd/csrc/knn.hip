@@ -206,12 +206,13 @@ __global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__rest
                                                         const int *__restrict__ cell_grp,
                                                         const double *__restrict__ Q,
                                                         const int *__restrict__ qgroup,
-                                                        const int *__restrict__ qorder, long M, int k,
+                                                        const int *__restrict__ qorder, long M,
+                                                        const int *__restrict__ mcount, int k,
                                                         int *__restrict__ idx_out, double *__restrict__ d2_out)
 {
 #pragma clang fp contract(off)
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= M) return;
+    if (t >= (mcount ? (long)*mcount : M)) return;
     const GridParams g = *gp;
     const long q = qorder ? (long)qorder[t] : t;
     if (!g.valid) {  // non-finite points: no search, every slot -1
@@ -306,30 +307,175 @@ __global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__rest
     }
 }
 
+// Two-phase exact selection for larger k.  Phase 1 keeps only the KMAX
+// smallest squared distances, without their indices, in a min/max exchange
+// chain (two instructions a slot where the insertion needs about nine), which
+// yields tau, the k-th smallest.  Phase 2 walks the same cells again and
+// writes every candidate with d <= tau at its rank among the kept values.
+// That rank is exact when the kept values are distinct and exactly k
+// candidates have d <= tau; any other query (an exact distance tie, fewer than
+// k visible nodes, non-finite points) goes to `redo`, which the insertion
+// kernel then recomputes with the (d^2, index) order.
+template <int KMAX>
+__global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__restrict__ gp,
+                                                         const int *__restrict__ cell_start,
+                                                         const double *__restrict__ cell_xy,
+                                                         const int *__restrict__ cell_idx,
+                                                         const int *__restrict__ cell_grp,
+                                                         const double *__restrict__ Q,
+                                                         const int *__restrict__ qgroup,
+                                                         const int *__restrict__ qorder, long M, int k,
+                                                         int *__restrict__ idx_out, double *__restrict__ d2_out,
+                                                         int *__restrict__ redo, int *__restrict__ redo_count)
+{
+#pragma clang fp contract(off)
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    const GridParams g = *gp;
+    const long q = qorder ? (long)qorder[t] : t;
+    if (!g.valid) {
+        redo[atomicAdd(redo_count, 1)] = (int)q;
+        return;
+    }
+    const double qx = Q[2 * q], qy = Q[2 * q + 1];
+    const int qg = (cell_grp && qgroup) ? qgroup[q] : -1;
+
+    // kept values ascending; slots [0, KMAX-k) hold a sentinel below any distance
+    double kv[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) kv[j] = j < KMAX - k ? -1.0 : INFINITY;
+
+    const double fx = (qx - g.x0) / g.csx, fy = (qy - g.y0) / g.csy;
+    const int qcx = (int)fmin(fmax(floor(fx), -2.0), (double)g.gx + 1.0);
+    const int qcy = (int)fmin(fmax(floor(fy), -2.0), (double)g.gy + 1.0);
+    const int rlimit = g.gx + g.gy + 4;
+
+    // the cells of ring r in a fixed order; f(cell, lower bound on d^2)
+    auto ring = [&](int r, auto &&f) {
+        const int ncell = r == 0 ? 1 : 8 * r;
+        for (int tc = 0; tc < ncell; ++tc) {
+            int dxc = 0, dyc = 0;
+            if (r > 0) {
+                int side = tc / (2 * r), off = tc - side * 2 * r;
+                if (side == 0) { dxc = -r + off; dyc = -r; }
+                else if (side == 1) { dxc = r; dyc = -r + off; }
+                else if (side == 2) { dxc = r - off; dyc = r; }
+                else { dxc = -r; dyc = r - off; }
+            }
+            const int cx = qcx + dxc, cy = qcy + dyc;
+            if (cx < 0 || cx >= g.gx || cy < 0 || cy >= g.gy) continue;
+            const double rx0 = g.x0 + cx * g.csx, rx1 = rx0 + g.csx;
+            const double ry0 = g.y0 + cy * g.csy, ry1 = ry0 + g.csy;
+            const double ddx = qx < rx0 ? rx0 - qx : (qx > rx1 ? qx - rx1 : 0.0);
+            const double ddy = qy < ry0 ? ry0 - qy : (qy > ry1 ? qy - ry1 : 0.0);
+            f(cy * g.gx + cx, (ddx * ddx + ddy * ddy) * (1.0 - 1e-9));
+        }
+    };
+    // squared distance to candidate p, or -1 when the query does not see it
+    auto dist = [&](int p) {
+        if (qg >= 0) {
+            const int h = cell_grp[p];
+            if (h != 0 && h != qg) return -1.0;
+        }
+        const double dx = cell_xy[2 * (long)p] - qx, dy = cell_xy[2 * (long)p + 1] - qy;
+        const double dd = dx * dx;
+        const double ee = dy * dy;
+        return dd + ee;
+    };
+
+    // ---- phase 1: the KMAX smallest values
+    int rlast = 0;
+    for (int r = 0; r <= rlimit; ++r) {
+        rlast = r;
+        ring(r, [&](int c, double md) {
+            if (md > kv[KMAX - 1]) return;
+            const int p1 = cell_start[c + 1];
+            for (int p = cell_start[c]; p < p1; ++p) {
+                const double d = dist(p);
+                if (d >= 0.0 && d < kv[KMAX - 1]) {
+                    double cv = d;
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j) {
+                        const double lo = __builtin_fmin(cv, kv[j]);
+                        cv = __builtin_fmax(cv, kv[j]);
+                        kv[j] = lo;
+                    }
+                }
+            }
+        });
+        const double bx0 = qx - (g.x0 + (double)(qcx - r) * g.csx);
+        const double bx1 = (g.x0 + (double)(qcx + r + 1) * g.csx) - qx;
+        const double by0 = qy - (g.y0 + (double)(qcy - r) * g.csy);
+        const double by1 = (g.y0 + (double)(qcy + r + 1) * g.csy) - qy;
+        const double b = fmin(fmin(bx0, bx1), fmin(by0, by1));
+        if (b > 0.0 && b * b * (1.0 - 1e-9) > kv[KMAX - 1]) break;
+        if (qcx - r <= 0 && qcy - r <= 0 && qcx + r >= g.gx - 1 && qcy + r >= g.gy - 1) break;
+    }
+
+    // ---- phase 2: every candidate with d <= tau at its rank
+    const double tau = kv[KMAX - 1];
+    bool tie = !(tau < INFINITY);
+#pragma unroll
+    for (int j = 1; j < KMAX; ++j)
+        if (j > KMAX - k) tie |= kv[j] == kv[j - 1];
+    int nsel = 0;
+    if (!tie) {
+        for (int r = 0; r <= rlast; ++r) {
+            ring(r, [&](int c, double md) {
+                if (md > tau) return;
+                const int p1 = cell_start[c + 1];
+                for (int p = cell_start[c]; p < p1; ++p) {
+                    const double d = dist(p);
+                    if (d >= 0.0 && d <= tau) {
+                        int pos = -(KMAX - k);
+#pragma unroll
+                        for (int j = 0; j < KMAX; ++j) pos += kv[j] < d ? 1 : 0;
+                        idx_out[q * k + pos] = cell_idx[p];
+                        if (d2_out) d2_out[q * k + pos] = d;
+                        ++nsel;
+                    }
+                }
+            });
+        }
+    }
+    if (tie || nsel != k) redo[atomicAdd(redo_count, 1)] = (int)q;
+}
+
 template <int KMAX>
 static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
                          const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
-                         double *d2, hipStream_t s)
+                         double *d2, int *redo, int *redo_count, hipStream_t s)
 {
-    knn_query_kernel<KMAX><<<(unsigned)((M + 255) / 256), 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup,
-                                                                         qorder, M, k, idx, d2);
+    const unsigned blocks = (unsigned)((M + 255) / 256);
+    if (KMAX >= 16 && redo) {
+        knn_select_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2,
+                                                        redo, redo_count);
+        knn_query_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, redo, M, redo_count, k,
+                                                       idx, d2);
+    } else {
+        knn_query_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, nullptr, k,
+                                                       idx, d2);
+    }
 }
 
 void launch_knn_query(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
                       const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
-                      double *d2, hipStream_t s)
+                      double *d2, int *redo, int *redo_count, hipStream_t s)
 {
     if (M <= 0) return;
-    if (k <= 1) knn_launch_t<1>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    else if (k <= 8) knn_launch_t<8>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    else if (k <= 16) knn_launch_t<16>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    else if (k <= 24) knn_launch_t<24>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    else if (k <= 32) knn_launch_t<32>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    else if (k <= 48) knn_launch_t<48>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    else if (k <= 64) knn_launch_t<64>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    else if (k <= 96) knn_launch_t<96>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
-    else if (k <= 128) knn_launch_t<128>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, s);
+    if (redo) launch_fill_i32(redo_count, 1, 0, s);
+#define RBFFD_KNN(KM) knn_launch_t<KM>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, redo, redo_count, s)
+    if (k <= 1) RBFFD_KNN(1);
+    else if (k <= 8) RBFFD_KNN(8);
+    else if (k <= 16) RBFFD_KNN(16);
+    else if (k <= 24) RBFFD_KNN(24);
+    else if (k <= 32) RBFFD_KNN(32);
+    else if (k <= 48) RBFFD_KNN(48);
+    else if (k <= 64) RBFFD_KNN(64);
+    else if (k <= 96) RBFFD_KNN(96);
+    else if (k <= 128) RBFFD_KNN(128);
     else throw Error{2, "knn: k > 128 is not supported"};
+#undef RBFFD_KNN
 }
 
 }  // namespace rbffd
