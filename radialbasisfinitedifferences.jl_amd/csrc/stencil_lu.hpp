@@ -394,8 +394,8 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     }
     __builtin_amdgcn_wave_barrier();
 
-    // ---- LU with partial pivoting, forward substitution in the RHS columns.
-    // Rows slide one place per step (row[j] <- row[j+1] - l u[j+1]), so the
+    // ---- Gauss-Jordan elimination with partial pivoting; the right-hand-side
+    // columns ride along.  Rows slide one place per step (row[j] <- row[j+1] - l u[j+1]), so the
     // pivot column is always register 0 and every register index is static.
     // Steps run as a real loop in blocks of SB steps with a static width that
     // shrinks per block: compact code, at most SB-1 stale columns of
