@@ -451,41 +451,56 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             const int v = 127 - (int)(kmax & 127u);
             const bool iam = t == v % G;
             const int psel = v / G;
-            double *piv = U + T::uoff(k);  // piv[j] = U(k, k + j)
+            double *piv = U + T::uoff(k);  // piv[j] = pivot row, column k + j
+            // The row goes out in two parts: columns 0..15 and the reciprocal
+            // pivot, read back at once, then the rest.  LDS serves a
+            // wavefront's operations in order, so the first reads and the
+            // column k+1 update need not wait for the whole row.
+            constexpr int W0 = WB < 16 ? WB : 16;
+            constexpr int NPV = (WB + 15) / 16;
             uint32_t cur = 0;
+            if (iam) L.rp[grp] = crp;
 #pragma unroll
             for (int rr = 0; rr < ROWS; ++rr) {
                 if (iam && psel == rr) {
 #pragma unroll
-                    for (int j = 0; j < WB; ++j) piv[j] = row[rr][j];
+                    for (int j = 0; j < W0; ++j) piv[j] = row[rr][j];
                     done |= 1u << rr;
                     cur = 1u << rr;
                     kp[rr] = k;
                     pval[rr] = row[rr][0];
                 }
             }
-            if (iam) L.rp[grp] = crp;
             // the other lanes read what the pivot lane just wrote
             __builtin_amdgcn_wave_barrier();
             const double rp = L.rp[grp];
+            const double p1 = piv[1];
             singular |= (piv[0] == 0.0);
             // the pivot row spread over each 16-lane DPP row: lane q holds
             // columns q, q+16, ...; a DPP broadcast hands column j to the FMAs
-            constexpr int NPV = (WB + 15) / 16;
             double pv[NPV];
+            pv[0] = piv[lane & 15];
+            if constexpr (NPV > 1) {
+                __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int i = 0; i < NPV; ++i) pv[i] = piv[(lane & 15) + 16 * i];
+                for (int rr = 0; rr < ROWS; ++rr) {
+                    if (iam && psel == rr) {
+#pragma unroll
+                        for (int j = 16; j < WB; ++j) piv[j] = row[rr][j];
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int i = 1; i < NPV; ++i) pv[i] = piv[(lane & 15) + 16 * i];
+            }
             double nl[ROWS];
-#pragma unroll
             // Gauss-Jordan: rows that were pivots earlier are eliminated too, so
             // no back substitution is needed (rows past the system are zero)
+#pragma unroll
             for (int rr = 0; rr < ROWS; ++rr) nl[rr] = ((cur >> rr) & 1u) ? -0.0 : -(row[rr][0] * rp);
             // column k+1 first: the next step's pivot search overlaps the rest
-            {
-                const double p1 = piv[1];
 #pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
-            }
+            for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
             candidate();
             kmax = group_max_u32<G>(key);
             for_blocks<1, WB - 1>([&](auto jc) {
