@@ -789,9 +789,21 @@ void lsq_impl(rbffd_context *c, long M, long N, const int64_t *rowptr, const int
         hipGraphExec_t exec = nullptr;
         if (st && max_iters >= kLsqChunk) {
             RBFFD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            for (long k = 0; k < kLsqChunk; ++k) lsq_iteration(S, k, st);
+            try {
+                for (long k = 0; k < kLsqChunk; ++k) lsq_iteration(S, k, st);
+                RBFFD_HIP(hipGetLastError());
+            } catch (...) {
+                // leave the stream out of capture mode
+                (void)hipStreamEndCapture(st, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
             RBFFD_HIP(hipStreamEndCapture(st, &graph));
-            RBFFD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                throw Error{RBFFD_ERR_RUNTIME, std::string("lsq: hipGraphInstantiate: ") + hipGetErrorString(e)};
+            }
         }
         try {
             while (done < max_iters && rel > tol) {
