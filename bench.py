@@ -6,7 +6,7 @@ nearest-node map, and the local solves of r^7 / degree-4 systems for the
 operators d/dx, d/dy and the Laplacian, written as FP64 CSR rows.
 
 The JSON line reports:
-  value                stencils/s of the whole step, FP64 LU solver (weights
+  value                stencils/s of the whole step, FP64 Gauss-Jordan solver (weights
                        within 1e-10 of the high-precision oracle)
   roofline             the local-solve kernel: algorithmic flops per launch
                        ((2/3) Nsys^3 + 2 Nsys^2 r per stencil, Nsys = 46, r = 3)

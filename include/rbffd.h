@@ -55,7 +55,7 @@ extern "C" {
 #define RBFFD_MAX_OPS 10
 
 /* precision of the local solves */
-#define RBFFD_PREC_FP64 0   /* FP64 LU with partial pivoting */
+#define RBFFD_PREC_FP64 0   /* FP64 Gauss-Jordan elimination with partial pivoting */
 #define RBFFD_PREC_MIXED 1  /* FP32 Gauss-Jordan inverse + FP64 refinement, FP64 re-solve where it
                                does not converge */
 #define RBFFD_PREC_AUTO 2   /* the faster of the two on this device: FP64 on MI355X, where FP32

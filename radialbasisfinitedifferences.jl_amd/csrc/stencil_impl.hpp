@@ -9,7 +9,7 @@
 // through LDS) gives an FP32 inverse and a first solution; FP64 refinement
 // x += inv (b - A x) then runs until the update reaches the FP64 floor or stops
 // shrinking.  Rows whose last update is above kAcceptRatio get status 3 and
-// are re-solved by the FP64 LU kernel (stencil_lu.hpp).
+// are re-solved by the FP64 kernel (stencil_lu.hpp).
 //
 // Weights are scaled as the reference does and written straight into CSR rows
 // whose columns are sorted (rank of each stencil index inside the stencil).
@@ -379,7 +379,7 @@ constexpr int lu_wpb(int bytes)
     return best;
 }
 
-// FP64 LU kernel: G lanes per stencil.
+// FP64 Gauss-Jordan kernel: G lanes per stencil.
 template <int NN, int NP, int R, int G>
 void launch_lu(const SolveArgs &a, hipStream_t s)
 {

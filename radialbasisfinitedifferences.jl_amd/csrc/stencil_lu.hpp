@@ -189,7 +189,8 @@ struct LuShape {
     static constexpr int WIDTH = NS + R;
     static constexpr int S = 64 / G;               // stencils per wavefront
     static constexpr int NODES = (NN + G - 1) / G; // stencil nodes per lane
-    // U factor, packed by pivot step: row k holds columns k..WIDTH-1
+    // pivot rows, packed by step: row k holds columns k..WIDTH-1 (each is read
+    // during its own step only)
     static constexpr int UTOT = NS * WIDTH - NS * (NS - 1) / 2;
     // during assembly: P (node-major), then Phi as a packed upper triangle
     static constexpr int PHI0 = NN * NP;
