@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--no-e2e", action="store_true", help="skip the host-level end-to-end leg")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other workloads (hyperviscosity 4M, uneven batch, time stepping)")
+    ap.add_argument("--extras", default="all",
+                    help="comma-separated subset of the extra workloads: hyperviscosity_4m, uneven_batch_64, "
+                         "advdiff_ssprk3_4m, poisson_solve (default all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1"}))
@@ -248,8 +251,11 @@ def main():
     if not args.no_extras:
         del col, vals, st_d, Xd
         extras = {}
+        wanted = None if args.extras == "all" else set(args.extras.split(","))
         for name, fn in (("hyperviscosity_4m", extra_hyper4m), ("uneven_batch_64", extra_batch),
                          ("advdiff_ssprk3_4m", extra_advdiff), ("poisson_solve", extra_poisson_solve)):
+            if wanted is not None and name not in wanted:
+                continue
             try:
                 extras[name] = fn(rbffd_amd, ctx, eng, args.no_cpu)
             except Exception as e:  # a failed extra is reported, the headline stands
