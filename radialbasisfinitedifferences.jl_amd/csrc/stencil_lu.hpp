@@ -74,8 +74,8 @@ __device__ __forceinline__ void for_blocks(F &&f)
 }
 
 // Maximum over each aligned group of G lanes; every lane gets its group's value.
-// DPP inside each 16-lane row, then the row maxima through scalar registers
-// (no LDS round trip).
+// DPP inside each 16-lane row, then lane swaps across rows and halves
+// (no LDS or scalar round trip).
 template <int G>
 __device__ __forceinline__ uint32_t group_max_u32(uint32_t v)
 {
@@ -85,13 +85,16 @@ __device__ __forceinline__ uint32_t group_max_u32(uint32_t v)
     v = max(v, dpp_u32<0x141>(v));  // row_half_mirror
     v = max(v, dpp_u32<0x140>(v));  // row_mirror
     if constexpr (G >= 32) {
-        const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
-        const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-        const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
-        const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-        const uint32_t lo = r0 > r1 ? r0 : r1, hi = r2 > r3 ? r2 : r3;
-        if constexpr (G == 64) v = lo > hi ? lo : hi;
-        else v = (threadIdx.x & 32) ? hi : lo;
+        // v_permlane16_swap exchanges the odd 16-lane rows of one copy with the
+        // even rows of the other: the two results hold rows (0, 0, 2, 2) and
+        // (1, 1, 3, 3), so their maximum is each 32-lane half's maximum
+        const auto sw = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = max((uint32_t)sw[0], (uint32_t)sw[1]);
+    }
+    if constexpr (G == 64) {
+        // v_permlane32_swap likewise gives halves (low, low) and (high, high)
+        const auto sw = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        v = max((uint32_t)sw[0], (uint32_t)sw[1]);
     }
     return v;
 }
