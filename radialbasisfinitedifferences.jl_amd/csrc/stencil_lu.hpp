@@ -64,6 +64,18 @@ __device__ __forceinline__ double row_bcast64(double v)
     return d;
 }
 
+// Two such broadcasts behind one wait: nothing inside the asm writes the
+// sources, and the outputs are early-clobber so neither aliases the second
+// source.
+template <int L0, int L1>
+__device__ __forceinline__ void row_bcast64x2(double v0, double v1, double &d0, double &d1)
+{
+    asm("s_nop 1\n\tv_mov_b64_dpp %0, %2 row_newbcast:%4 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b64_dpp %1, %3 row_newbcast:%5 row_mask:0xf bank_mask:0xf"
+        : "=&v"(d0), "=&v"(d1)
+        : "v"(v0), "v"(v1), "i"(L0), "i"(L1));
+}
+
 template <int B, int NB, class F>
 __device__ __forceinline__ void for_blocks(F &&f)
 {
@@ -507,12 +519,23 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
             candidate();
             kmax = group_max_u32<G>(key);
-            for_blocks<1, WB - 1>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
+            // columns j = 1 .. WB-2 in pairs (one wait for two broadcasts)
+            constexpr int NCOL = WB - 2;
+            for_blocks<0, NCOL / 2>([&](auto pc) {
+                constexpr int j = 1 + 2 * decltype(pc)::value;
+                double p0, p1;
+                row_bcast64x2<(j + 1) % 16, (j + 2) % 16>(pv[(j + 1) / 16], pv[(j + 2) / 16], p0, p1);
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
+            });
+            if constexpr (NCOL % 2 == 1) {
+                constexpr int j = NCOL;
                 const double pj = row_bcast64<(j + 1) % 16>(pv[(j + 1) / 16]);
 #pragma unroll
                 for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], pj, row[rr][j + 1]);
-            });
+            }
             // the next step's pivot lane overwrites the stale tail read here
             __builtin_amdgcn_wave_barrier();
         }
