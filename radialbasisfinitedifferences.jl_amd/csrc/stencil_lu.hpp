@@ -76,6 +76,18 @@ __device__ __forceinline__ void row_bcast64x2(double v0, double v1, double &d0, 
         : "v"(v0), "v"(v1), "i"(L0), "i"(L1));
 }
 
+template <int L0, int L1, int L2, int L3>
+__device__ __forceinline__ void row_bcast64x4(double v0, double v1, double v2, double v3, double &d0, double &d1,
+                                              double &d2, double &d3)
+{
+    asm("s_nop 1\n\tv_mov_b64_dpp %0, %4 row_newbcast:%8 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b64_dpp %1, %5 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b64_dpp %2, %6 row_newbcast:%10 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b64_dpp %3, %7 row_newbcast:%11 row_mask:0xf bank_mask:0xf"
+        : "=&v"(d0), "=&v"(d1), "=&v"(d2), "=&v"(d3)
+        : "v"(v0), "v"(v1), "v"(v2), "v"(v3), "i"(L0), "i"(L1), "i"(L2), "i"(L3));
+}
+
 template <int B, int NB, class F>
 __device__ __forceinline__ void for_blocks(F &&f)
 {
@@ -519,17 +531,33 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
             candidate();
             kmax = group_max_u32<G>(key);
-            // columns j = 1 .. WB-2 in pairs (one wait for two broadcasts)
+            // columns j = 1 .. WB-2 in fours (one wait for four broadcasts),
+            // then a pair and a single for the rest
             constexpr int NCOL = WB - 2;
-            for_blocks<0, NCOL / 2>([&](auto pc) {
-                constexpr int j = 1 + 2 * decltype(pc)::value;
+            for_blocks<0, NCOL / 4>([&](auto qc) {
+                constexpr int j = 1 + 4 * decltype(qc)::value;
+                double p0, p1, p2, p3;
+                row_bcast64x4<(j + 1) % 16, (j + 2) % 16, (j + 3) % 16, (j + 4) % 16>(
+                    pv[(j + 1) / 16], pv[(j + 2) / 16], pv[(j + 3) / 16], pv[(j + 4) / 16], p0, p1, p2, p3);
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 2] = fma3(nl[rr], p2, row[rr][j + 3]);
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 3] = fma3(nl[rr], p3, row[rr][j + 4]);
+            });
+            constexpr int J2 = 1 + 4 * (NCOL / 4);  // first column after the fours
+            if constexpr (NCOL % 4 >= 2) {
+                constexpr int j = J2;
                 double p0, p1;
                 row_bcast64x2<(j + 1) % 16, (j + 2) % 16>(pv[(j + 1) / 16], pv[(j + 2) / 16], p0, p1);
 #pragma unroll
                 for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
 #pragma unroll
                 for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
-            });
+            }
             if constexpr (NCOL % 2 == 1) {
                 constexpr int j = NCOL;
                 const double pj = row_bcast64<(j + 1) % 16>(pv[(j + 1) / 16]);
