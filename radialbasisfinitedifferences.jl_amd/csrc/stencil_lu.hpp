@@ -123,14 +123,33 @@ __device__ __forceinline__ uint32_t group_max_u32(uint32_t v)
     return v;
 }
 
+// Maximum of non-negative doubles over each aligned group of G lanes (their
+// bit patterns order like unsigned integers): DPP inside 16-lane rows, then
+// lane swaps across rows and halves, as group_max_u32.
+__device__ __forceinline__ uint64_t max_u64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
 template <int G>
 __device__ __forceinline__ double group_max_nonneg_f64(double v)
 {
     uint64_t b = (uint64_t)__double_as_longlong(v);
-#pragma unroll
-    for (int o = G / 2; o >= 1; o >>= 1) {
-        uint64_t ob = (uint64_t)__shfl_xor((unsigned long long)b, o, G);
-        b = ob > b ? ob : b;
+    auto dpp_step = [&](auto ctrl) {
+        constexpr int C = decltype(ctrl)::value;
+        const uint32_t lo = dpp_u32<C>((uint32_t)b), hi = dpp_u32<C>((uint32_t)(b >> 32));
+        b = max_u64(b, ((uint64_t)hi << 32) | lo);
+    };
+    dpp_step(std::integral_constant<int, 0xB1>{});   // quad_perm [1,0,3,2]
+    dpp_step(std::integral_constant<int, 0x4E>{});   // quad_perm [2,3,0,1]
+    dpp_step(std::integral_constant<int, 0x141>{});  // row_half_mirror
+    dpp_step(std::integral_constant<int, 0x140>{});  // row_mirror
+    if constexpr (G >= 32) {
+        const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)b, (uint32_t)b, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(b >> 32), (uint32_t)(b >> 32), false, false);
+        b = max_u64(((uint64_t)(uint32_t)hi[0] << 32) | (uint32_t)lo[0], ((uint64_t)(uint32_t)hi[1] << 32) | (uint32_t)lo[1]);
+    }
+    if constexpr (G == 64) {
+        const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)b, (uint32_t)b, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(b >> 32), (uint32_t)(b >> 32), false, false);
+        b = max_u64(((uint64_t)(uint32_t)hi[0] << 32) | (uint32_t)lo[0], ((uint64_t)(uint32_t)hi[1] << 32) | (uint32_t)lo[1]);
     }
     return __longlong_as_double((long long)b);
 }
