@@ -447,37 +447,38 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     // Steps run as a real loop in blocks of SB steps with a static width that
     // shrinks per block: compact code, at most SB-1 stale columns of
     // extra work per step.
-    uint32_t done = 0;
-#pragma unroll
-    for (int rr = 0; rr < ROWS; ++rr)
-        if (t + G * rr >= NS) done |= 1u << rr;
+    // alive[rr]: all ones while row slot rr holds an equation not yet used as a
+    // pivot, zero afterwards and for the slots past the system
+    uint32_t alive[ROWS];
     // the column each row was the pivot for, and that pivot
     int kp[ROWS];
     double pval[ROWS];
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
+        alive[rr] = t + G * rr < NS ? 0xFFFFFFFFu : 0u;
         kp[rr] = NS;
         pval[rr] = 1.0;
     }
-    bool singular = false;
     double *U = L.u[grp];
-    // This lane's pivot candidate: its largest |row[.][0]| among rows not yet
-    // eliminated, as a key whose low 7 bits name the row, and the reciprocal of
-    // that entry, so the winning lane publishes 1/pivot with the pivot row.
+    // This lane's pivot candidate: its largest |row[.][0]| among live rows, as
+    // a key whose low 7 bits name the row (bit 7 set, so a live zero still
+    // beats a used row), and the reciprocal of that entry, so the winning lane
+    // publishes 1/pivot with the pivot row.
     uint32_t key;
     double crp;
     auto candidate = [&]() {
         uint32_t kk = 0;
-        double cv = 1.0;
+        double cv = row[0][0];
 #pragma unroll
         for (int rr = 0; rr < ROWS; ++rr) {
-            if (!((done >> rr) & 1u)) {
-                const float f = __builtin_fabsf((float)row[rr][0]);
-                const uint32_t kr = (__float_as_uint(f) & 0xFFFFFF00u) | 0x80u | (uint32_t)(127 - (t + G * rr));
-                if (kr > kk) {
-                    kk = kr;
-                    cv = row[rr][0];
-                }
+            // |f| and the low byte cleared in one mask
+            const uint32_t kr = ((__float_as_uint((float)row[rr][0]) & 0x7FFFFF00u) |
+                                 (0x80u | (uint32_t)(127 - (t + G * rr)))) & alive[rr];
+            if (rr == 0) {
+                kk = kr;
+            } else if (kr > kk) {
+                kk = kr;
+                cv = row[rr][0];
             }
         }
         double r = __builtin_amdgcn_rcp(cv);
@@ -500,33 +501,85 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             const int psel = v / G;
             double *piv = U + T::uoff(k);  // piv[j] = pivot row, column k + j
             // The row goes out in two parts: columns 0..15 and the reciprocal
-            // pivot, read back at once, then the rest.  LDS serves a
-            // wavefront's operations in order, so the first reads and the
-            // column k+1 update need not wait for the whole row.
+            // pivot, read back at once; the rest after the update of the
+            // columns the first part feeds.  LDS serves a wavefront's
+            // operations in order, so those FMAs wait for the first part only.
             constexpr int W0 = WB < 16 ? WB : 16;
             constexpr int NPV = (WB + 15) / 16;
-            uint32_t cur = 0;
             if (iam) L.rp[grp] = crp;
 #pragma unroll
             for (int rr = 0; rr < ROWS; ++rr) {
                 if (iam && psel == rr) {
 #pragma unroll
                     for (int j = 0; j < W0; ++j) piv[j] = row[rr][j];
-                    done |= 1u << rr;
-                    cur = 1u << rr;
+                    alive[rr] = 0u;
                     kp[rr] = k;
                     pval[rr] = row[rr][0];
+                    // a zero multiplier keeps the pivot row as it is
+                    row[rr][0] = 0.0;
                 }
             }
             // the other lanes read what the pivot lane just wrote
             __builtin_amdgcn_wave_barrier();
             const double rp = L.rp[grp];
             const double p1 = piv[1];
-            singular |= (piv[0] == 0.0);
             // the pivot row spread over each 16-lane DPP row: lane q holds
             // columns q, q+16, ...; a DPP broadcast hands column j to the FMAs
             double pv[NPV];
             pv[0] = piv[lane & 15];
+            double nl[ROWS];
+            // Gauss-Jordan: rows that were pivots earlier are eliminated too, so
+            // no back substitution is needed (rows past the system are zero)
+#pragma unroll
+            for (int rr = 0; rr < ROWS; ++rr) nl[rr] = -(row[rr][0] * rp);
+            // column k+1 first: the next step's pivot search overlaps the rest
+#pragma unroll
+            for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
+            candidate();
+            kmax = group_max_u32<G>(key);
+            // columns j = J0 .. J1-1 get column j+1 of the pivot row: in fours
+            // (one wait for four broadcasts), then a pair and a single
+            auto columns = [&](auto j0c, auto j1c) {
+                constexpr int J0 = decltype(j0c)::value, J1 = decltype(j1c)::value;
+                constexpr int NQ = (J1 - J0) / 4;
+                for_blocks<0, NQ>([&](auto qc) {
+                    constexpr int j = J0 + 4 * decltype(qc)::value;
+                    double p0, p1, p2, p3;
+                    row_bcast64x4<(j + 1) % 16, (j + 2) % 16, (j + 3) % 16, (j + 4) % 16>(
+                        pv[(j + 1) / 16], pv[(j + 2) / 16], pv[(j + 3) / 16], pv[(j + 4) / 16], p0, p1, p2, p3);
+#pragma unroll
+                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
+#pragma unroll
+                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
+#pragma unroll
+                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 2] = fma3(nl[rr], p2, row[rr][j + 3]);
+#pragma unroll
+                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 3] = fma3(nl[rr], p3, row[rr][j + 4]);
+                });
+                constexpr int J2 = J0 + 4 * NQ;
+                if constexpr (J1 - J2 >= 2) {
+                    constexpr int j = J2;
+                    double p0, p1;
+                    row_bcast64x2<(j + 1) % 16, (j + 2) % 16>(pv[(j + 1) / 16], pv[(j + 2) / 16], p0, p1);
+#pragma unroll
+                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
+#pragma unroll
+                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
+                }
+                if constexpr ((J1 - J2) % 2 == 1) {
+                    constexpr int j = J1 - 1;
+                    const double pj = row_bcast64<(j + 1) % 16>(pv[(j + 1) / 16]);
+#pragma unroll
+                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], pj, row[rr][j + 1]);
+                }
+            };
+            // columns up to 15 come from the first part; the rest of the row
+            // goes out only now, so the wait for the first part does not
+            // cover it.  The pivot lane's registers from 16 on are not yet
+            // updated (the update reaches register j from j+1 = 16 next).
+            constexpr int JE = WB - 1;  // columns j = 1 .. WB-2
+            constexpr int JA = JE < 15 ? JE : 15;
+            columns(std::integral_constant<int, 1>{}, std::integral_constant<int, JA>{});
             if constexpr (NPV > 1) {
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -539,49 +592,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int i = 1; i < NPV; ++i) pv[i] = piv[(lane & 15) + 16 * i];
-            }
-            double nl[ROWS];
-            // Gauss-Jordan: rows that were pivots earlier are eliminated too, so
-            // no back substitution is needed (rows past the system are zero)
-#pragma unroll
-            for (int rr = 0; rr < ROWS; ++rr) nl[rr] = ((cur >> rr) & 1u) ? -0.0 : -(row[rr][0] * rp);
-            // column k+1 first: the next step's pivot search overlaps the rest
-#pragma unroll
-            for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
-            candidate();
-            kmax = group_max_u32<G>(key);
-            // columns j = 1 .. WB-2 in fours (one wait for four broadcasts),
-            // then a pair and a single for the rest
-            constexpr int NCOL = WB - 2;
-            for_blocks<0, NCOL / 4>([&](auto qc) {
-                constexpr int j = 1 + 4 * decltype(qc)::value;
-                double p0, p1, p2, p3;
-                row_bcast64x4<(j + 1) % 16, (j + 2) % 16, (j + 3) % 16, (j + 4) % 16>(
-                    pv[(j + 1) / 16], pv[(j + 2) / 16], pv[(j + 3) / 16], pv[(j + 4) / 16], p0, p1, p2, p3);
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 2] = fma3(nl[rr], p2, row[rr][j + 3]);
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 3] = fma3(nl[rr], p3, row[rr][j + 4]);
-            });
-            constexpr int J2 = 1 + 4 * (NCOL / 4);  // first column after the fours
-            if constexpr (NCOL % 4 >= 2) {
-                constexpr int j = J2;
-                double p0, p1;
-                row_bcast64x2<(j + 1) % 16, (j + 2) % 16>(pv[(j + 1) / 16], pv[(j + 2) / 16], p0, p1);
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
-            }
-            if constexpr (NCOL % 2 == 1) {
-                constexpr int j = NCOL;
-                const double pj = row_bcast64<(j + 1) % 16>(pv[(j + 1) / 16]);
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], pj, row[rr][j + 1]);
+                columns(std::integral_constant<int, JA>{}, std::integral_constant<int, JE>{});
             }
             // the next step's pivot lane overwrites the stale tail read here
             __builtin_amdgcn_wave_barrier();
@@ -623,6 +634,10 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             }
         }
     }
+    // a zero pivot: the system is exactly singular (rows past the system keep 1)
+    bool singular = false;
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr) singular |= pval[rr] == 0.0;
     const uint32_t sflag = group_max_u32<G>((singular ? 2u : 0u) | bad);
     if (a.status && t == 0 && live) {
         const int st = (sflag & 2u) ? 1 : ((sflag & 1u) ? 2 : 0);
