@@ -235,22 +235,33 @@ struct LuShape {
     static constexpr int WIDTH = NS + R;
     static constexpr int S = 64 / G;               // stencils per wavefront
     static constexpr int NODES = (NN + G - 1) / G; // stencil nodes per lane
-    // pivot rows, packed by step: row k holds columns k..WIDTH-1 (each is read
-    // during its own step only)
-    static constexpr int UTOT = NS * WIDTH - NS * (NS - 1) / 2;
-    // during assembly: P (node-major), then Phi as a packed upper triangle
-    static constexpr int PHI0 = NN * NP;
-    // scaled node coordinates (x, y) pairs after Phi, until elimination starts
+    static constexpr int SB = step_block<G>();
+    // Monomial columns first: when the RBF equations fill whole row slots of
+    // their own (slots 0..R1-1) and the monomial equations the rest, the
+    // lambda columns are eliminated first, highest degree first.  The
+    // monomial equations are zero in those columns, so for the first NP steps
+    // their slots are neither updated nor searched, and they are loaded only
+    // when step NP-1 begins.
+    static constexpr int R1 = (NN + G - 1) / G;
+    static constexpr bool PFIRST = R1 + (NP + G - 1) / G == ROWS;
+    // the current pivot row: one buffer, reused by every step (LDS serves a
+    // wavefront's operations in order, so a step's reads finish before the
+    // next step's writes).  A step writes up to SB-1 stale values past its
+    // row and reads the row in whole runs of 16.
+    static constexpr int UBUF = 16 * ((WIDTH + 15) / 16);
+    // during assembly: Phi as a packed upper triangle and the scaled node
+    // coordinates (x, y) pairs.  P (node-major) follows them when the P^T
+    // rows are built with the others; with PFIRST it is written once the rows
+    // are built, over the dead triangle past the pivot-row buffer, and read
+    // when the monomial equations are loaded.
+    static constexpr int PHI0 = 0;
     static constexpr int XI0 = PHI0 + NN * (NN + 1) / 2;
-    static constexpr int ASM = XI0 + 2 * NN;
+    static constexpr int PB0 = PFIRST ? UBUF : XI0 + 2 * NN;
+    static constexpr int ASM = cmax_<XI0 + 2 * NN, PB0 + NN * NP>::v;
     // Phi(a, b), a <= b, at PHI0 + tri(a) + b
     static constexpr __host__ __device__ int tri(int a) { return a * (2 * NN - a + 1) / 2 - a; }
-    static constexpr int SB = step_block<G>();
-    // +SB+16: a step writes up to SB-1 stale values past its row and reads
-    // its row in whole runs of 16; +2 keeps the groups' slots off one LDS bank
-    static constexpr int USTRIDE = cmax_<UTOT, ASM>::v + SB + 16 + 2;
-    static constexpr int NBLK = (NS + SB - 1) / SB;
-    static constexpr __host__ __device__ int uoff(int k) { return k * WIDTH - k * (k - 1) / 2; }
+    // +2 keeps the groups' slots off one LDS bank
+    static constexpr int USTRIDE = cmax_<UBUF, ASM>::v + 2;
 };
 
 template <int NN, int NP, int R, int G>
@@ -337,10 +348,11 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             }
         }
     }
-    // monomial block P (node-major) for the transposed rows
+    // monomial block P (node-major) for the transposed rows; with PFIRST the
+    // rows are built first and P follows from their registers
     constexpr int DEG = mono_deg(NP - 1);
 #pragma unroll
-    for (int r = 0; r < NODES; ++r) {
+    for (int r = 0; r < (T::PFIRST ? 0 : NODES); ++r) {
         const int i = t + G * r;
         if (i < NN) {
             double pxp[DEG + 1], pyp[DEG + 1];
@@ -349,7 +361,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
 #pragma unroll
             for (int d = 1; d <= DEG; ++d) { pxp[d] = pxp[d - 1] * dxn[r]; pyp[d] = pyp[d - 1] * dyn[r]; }
 #pragma unroll
-            for (int q = 0; q < NP; ++q) L.u[grp][i * NP + q] = pxp[mono_ea(q)] * pyp[mono_eb(q)];
+            for (int q = 0; q < NP; ++q) L.u[grp][T::PB0 + i * NP + q] = pxp[mono_ea(q)] * pyp[mono_eb(q)];
         }
     }
     // Phi_ab = |xi_a - xi_b|^m once per pair: node i takes the pairs (i, i+d mod n)
@@ -375,16 +387,34 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     }
     __builtin_amdgcn_wave_barrier();
 
+    // ---- equations by row slot.  Lane t's slot rr holds RBF equation i
+    // (src/rbfblock.jl:14-20 with the P block) or monomial equation q (the
+    // P^T rows); eq_of gives i, NN + q, or NS for an empty slot.
+    auto eq_of = [&](int rr) -> int {
+        if constexpr (T::PFIRST) {
+            if (rr < T::R1) {
+                const int i = t + G * rr;
+                return i < NN ? i : NS;
+            }
+            const int q = t + G * (rr - T::R1);
+            return q < NP ? NN + q : NS;
+        } else {
+            const int i = t + G * rr;
+            return i < NS ? i : NS;
+        }
+    };
+
     // ---- right-hand sides first, while few registers are live: RBF rows at
     // u = Yscaled - xi with exact zeros -> eps (src/generate_operator.jl:125-154),
     // polynomial rows at Yscaled (:68-79)
     double rhs[ROWS][R];
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
-        const int i = t + G * rr;
+        const int e = eq_of(rr);
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) rhs[rr][cc] = 0.0;
-        if (i < NN) {
+        if (e < NN) {
+            const int i = e;
             const double xix = L.u[grp][T::XI0 + 2 * i], xiy = L.u[grp][T::XI0 + 2 * i + 1];
             double ux, uy;
             {
@@ -403,8 +433,8 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
 #pragma unroll
             for (int cc = 0; cc < R; ++cc)
                 if (cc < a.nops) rhs[rr][cc] = op_phs_fast(a.op_kind[cc], a.op_k[cc], m, ux, uy, sx, sy, q);
-        } else if (i < NS) {
-            const int q = i - NN;
+        } else if (e < NS) {
+            const int q = e - NN;
             int tt = 0;
             while ((tt + 1) * (tt + 2) / 2 <= q) ++tt;
             const int ea = q - tt * (tt + 1) / 2, eb = tt - ea;
@@ -414,24 +444,40 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         }
     }
 
-    // ---- rows of [A | B] (src/interpolationmatrix.jl:2-5, src/generate_operator.jl:157)
+    // ---- rows of [A | B] (src/interpolationmatrix.jl:2-5, src/generate_operator.jl:157).
+    // Column order: w then lambda; with PFIRST lambda (highest degree first)
+    // then w.  Either way the right-hand sides follow at NS.
     double row[ROWS][WIDTH];
+    const double *phi = L.u[grp] + T::PHI0;
+    const double *pblk = L.u[grp] + T::PB0;  // P(node j, monomial q) at j * NP + q
 #pragma unroll
-    for (int rr = 0; rr < ROWS; ++rr) {
-        const int i = t + G * rr;
-        if (i < NN) {
-            const double *phi = L.u[grp] + T::PHI0;
-            const int rbase = T::tri(i);  // Phi(i, j) for j >= i
+    for (int rr = 0; rr < (T::PFIRST ? T::R1 : ROWS); ++rr) {
+        const int e = eq_of(rr);
+        constexpr int W0 = T::PFIRST ? NP : 0;  // first w column
+        constexpr int L0 = T::PFIRST ? 0 : NN;  // first lambda column
+        if (e < NN) {
+            const int rbase = T::tri(e);  // Phi(e, j) for j >= e
 #pragma unroll
-            for (int j = 0; j < NN; ++j) row[rr][j] = phi[j >= i ? rbase + j : T::tri(j) + i];
+            for (int j = 0; j < NN; ++j) row[rr][W0 + j] = phi[j >= e ? rbase + j : T::tri(j) + e];
+            if constexpr (T::PFIRST) {
+                // node e = t + G rr is this lane's node rr: its monomials, highest first
+                double pxp[DEG + 1], pyp[DEG + 1];
+                pxp[0] = 1.0;
+                pyp[0] = 1.0;
 #pragma unroll
-            for (int q = 0; q < NP; ++q) row[rr][NN + q] = L.u[grp][i * NP + q];
-        } else if (i < NS) {
-            const int q = i - NN;
+                for (int d = 1; d <= DEG; ++d) { pxp[d] = pxp[d - 1] * dxn[rr]; pyp[d] = pyp[d - 1] * dyn[rr]; }
 #pragma unroll
-            for (int j = 0; j < NN; ++j) row[rr][j] = L.u[grp][j * NP + q];
+                for (int q = 0; q < NP; ++q) row[rr][q] = pxp[mono_ea(NP - 1 - q)] * pyp[mono_eb(NP - 1 - q)];
+            } else {
 #pragma unroll
-            for (int j = NN; j < NS; ++j) row[rr][j] = 0.0;
+                for (int q = 0; q < NP; ++q) row[rr][L0 + q] = pblk[e * NP + q];
+            }
+        } else if (e < NS) {
+            const int q = e - NN;
+#pragma unroll
+            for (int j = 0; j < NN; ++j) row[rr][W0 + j] = pblk[j * NP + q];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) row[rr][L0 + j] = 0.0;
         } else {
 #pragma unroll
             for (int j = 0; j < NS; ++j) row[rr][j] = 0.0;
@@ -439,6 +485,36 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) row[rr][NS + cc] = rhs[rr][cc];
     }
+    if constexpr (T::PFIRST) {
+        // P for the monomial equations, over the triangle just read
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int rr = 0; rr < T::R1; ++rr) {
+            const int e = eq_of(rr);
+            if (e < NN) {
+#pragma unroll
+                for (int q = 0; q < NP; ++q) L.u[grp][T::PB0 + e * NP + (NP - 1 - q)] = row[rr][q];
+            }
+        }
+    }
+    // PFIRST: the monomial equations as they stand after NP slides, column
+    // NP + j in register j (they are zero in the lambda columns, which the
+    // first NP steps eliminate without touching them)
+    auto load_monomial_rows = [&]() {
+#pragma unroll
+        for (int rr = T::R1; rr < ROWS; ++rr) {
+            const int e = eq_of(rr);
+            if (e < NS) {
+#pragma unroll
+                for (int j = 0; j < NN; ++j) row[rr][j] = pblk[j * NP + (e - NN)];
+            } else {
+#pragma unroll
+                for (int j = 0; j < NN; ++j) row[rr][j] = 0.0;
+            }
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) row[rr][NN + cc] = rhs[rr][cc];
+        }
+    };
     __builtin_amdgcn_wave_barrier();
 
     // ---- Gauss-Jordan elimination with partial pivoting; the right-hand-side
@@ -448,29 +524,30 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     // shrinks per block: compact code, at most SB-1 stale columns of
     // extra work per step.
     // alive[rr]: all ones while row slot rr holds an equation not yet used as a
-    // pivot, zero afterwards and for the slots past the system
+    // pivot, zero afterwards and for empty slots
     uint32_t alive[ROWS];
     // the column each row was the pivot for, and that pivot
     int kp[ROWS];
     double pval[ROWS];
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
-        alive[rr] = t + G * rr < NS ? 0xFFFFFFFFu : 0u;
+        alive[rr] = eq_of(rr) < NS ? 0xFFFFFFFFu : 0u;
         kp[rr] = NS;
         pval[rr] = 1.0;
     }
     double *U = L.u[grp];
-    // This lane's pivot candidate: its largest |row[.][0]| among live rows, as
-    // a key whose low 7 bits name the row (bit 7 set, so a live zero still
-    // beats a used row), and the reciprocal of that entry, so the winning lane
-    // publishes 1/pivot with the pivot row.
+    // This lane's pivot candidate among its first RC slots: the largest
+    // |row[.][0]| among live rows, as a key whose low 7 bits name the row slot
+    // (bit 7 set, so a live zero still beats a used row), and the reciprocal
+    // of that entry, so the winning lane publishes 1/pivot with the pivot row.
     uint32_t key;
     double crp;
-    auto candidate = [&]() {
+    auto candidate = [&](auto rcc) {
+        constexpr int RC = decltype(rcc)::value;
         uint32_t kk = 0;
         double cv = row[0][0];
 #pragma unroll
-        for (int rr = 0; rr < ROWS; ++rr) {
+        for (int rr = 0; rr < RC; ++rr) {
             // |f| and the low byte cleared in one mask
             const uint32_t kr = ((__float_as_uint((float)row[rr][0]) & 0x7FFFFF00u) |
                                  (0x80u | (uint32_t)(127 - (t + G * rr)))) & alive[rr];
@@ -487,117 +564,140 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         key = kk;
         crp = r;
     };
-    candidate();
+    using RowsPhase1 = std::integral_constant<int, T::PFIRST ? T::R1 : ROWS>;
+    using RowsAll = std::integral_constant<int, ROWS>;
+    candidate(RowsPhase1{});
     uint32_t kmax = group_max_u32<G>(key);
     int k = 0;
-    for_blocks<0, T::NBLK>([&](auto bc) {
-        constexpr int b = decltype(bc)::value;
-        constexpr int WB = WIDTH - T::SB * b;  // >= WIDTH - k for every k of the block
-        const int kend = NS < T::SB * (b + 1) ? NS : T::SB * (b + 1);
-#pragma nounroll
-        for (; k < kend; ++k) {
-            const int v = 127 - (int)(kmax & 127u);
-            const bool iam = t == v % G;
-            const int psel = v / G;
-            double *piv = U + T::uoff(k);  // piv[j] = pivot row, column k + j
-            // The row goes out in two parts: columns 0..15 and the reciprocal
-            // pivot, read back at once; the rest after the update of the
-            // columns the first part feeds.  LDS serves a wavefront's
-            // operations in order, so those FMAs wait for the first part only.
-            constexpr int W0 = WB < 16 ? WB : 16;
-            constexpr int NPV = (WB + 15) / 16;
-            if (iam) L.rp[grp] = crp;
+    // One step at static width WB >= WIDTH - k: slots below RF are updated
+    // (the winner is among them), slots below RC enter the next pivot search.
+    auto step = [&](auto wbc, auto rfc, auto rcc) {
+        constexpr int WB = decltype(wbc)::value;
+        constexpr int RF = decltype(rfc)::value;
+        const int v = 127 - (int)(kmax & 127u);
+        const bool iam = t == v % G;
+        const int psel = v / G;
+        double *piv = U;  // piv[j] = pivot row, column k + j
+        // The row goes out in two parts: columns 0..15 and the reciprocal
+        // pivot, read back at once; the rest after the update of the
+        // columns the first part feeds.  LDS serves a wavefront's
+        // operations in order, so those FMAs wait for the first part only.
+        constexpr int W0 = WB < 16 ? WB : 16;
+        constexpr int NPV = (WB + 15) / 16;
+        if (iam) L.rp[grp] = crp;
 #pragma unroll
-            for (int rr = 0; rr < ROWS; ++rr) {
+        for (int rr = 0; rr < RF; ++rr) {
+            if (iam && psel == rr) {
+#pragma unroll
+                for (int j = 0; j < W0; ++j) piv[j] = row[rr][j];
+                alive[rr] = 0u;
+                kp[rr] = k;
+                pval[rr] = row[rr][0];
+                // a zero multiplier keeps the pivot row as it is
+                row[rr][0] = 0.0;
+            }
+        }
+        // the other lanes read what the pivot lane just wrote
+        __builtin_amdgcn_wave_barrier();
+        const double rp = L.rp[grp];
+        const double p1 = piv[1];
+        // the pivot row spread over each 16-lane DPP row: lane q holds
+        // columns q, q+16, ...; a DPP broadcast hands column j to the FMAs
+        double pv[NPV];
+        pv[0] = piv[lane & 15];
+        double nl[ROWS];
+        // Gauss-Jordan: rows that were pivots earlier are eliminated too, so
+        // no back substitution is needed (rows past the system are zero)
+#pragma unroll
+        for (int rr = 0; rr < RF; ++rr) nl[rr] = -(row[rr][0] * rp);
+        // column k+1 first: the next step's pivot search overlaps the rest
+#pragma unroll
+        for (int rr = 0; rr < RF; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
+        candidate(rcc);
+        kmax = group_max_u32<G>(key);
+        // columns j = J0 .. J1-1 get column j+1 of the pivot row: in fours
+        // (one wait for four broadcasts), then a pair and a single
+        auto columns = [&](auto j0c, auto j1c) {
+            constexpr int J0 = decltype(j0c)::value, J1 = decltype(j1c)::value;
+            constexpr int NQ = (J1 - J0) / 4;
+            for_blocks<0, NQ>([&](auto qc) {
+                constexpr int j = J0 + 4 * decltype(qc)::value;
+                double p0, p1, p2, p3;
+                row_bcast64x4<(j + 1) % 16, (j + 2) % 16, (j + 3) % 16, (j + 4) % 16>(
+                    pv[(j + 1) / 16], pv[(j + 2) / 16], pv[(j + 3) / 16], pv[(j + 4) / 16], p0, p1, p2, p3);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr) row[rr][j + 2] = fma3(nl[rr], p2, row[rr][j + 3]);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr) row[rr][j + 3] = fma3(nl[rr], p3, row[rr][j + 4]);
+            });
+            constexpr int J2 = J0 + 4 * NQ;
+            if constexpr (J1 - J2 >= 2) {
+                constexpr int j = J2;
+                double p0, p1;
+                row_bcast64x2<(j + 1) % 16, (j + 2) % 16>(pv[(j + 1) / 16], pv[(j + 2) / 16], p0, p1);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
+            }
+            if constexpr ((J1 - J2) % 2 == 1) {
+                constexpr int j = J1 - 1;
+                const double pj = row_bcast64<(j + 1) % 16>(pv[(j + 1) / 16]);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr) row[rr][j] = fma3(nl[rr], pj, row[rr][j + 1]);
+            }
+        };
+        // columns up to 15 come from the first part; the rest of the row
+        // goes out only now, so the wait for the first part does not
+        // cover it.  The pivot lane's registers from 16 on are not yet
+        // updated (the update reaches register j from j+1 = 16 next).
+        constexpr int JE = WB - 1;  // columns j = 1 .. WB-2
+        constexpr int JA = JE < 15 ? JE : 15;
+        columns(std::integral_constant<int, 1>{}, std::integral_constant<int, JA>{});
+        if constexpr (NPV > 1) {
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int rr = 0; rr < RF; ++rr) {
                 if (iam && psel == rr) {
 #pragma unroll
-                    for (int j = 0; j < W0; ++j) piv[j] = row[rr][j];
-                    alive[rr] = 0u;
-                    kp[rr] = k;
-                    pval[rr] = row[rr][0];
-                    // a zero multiplier keeps the pivot row as it is
-                    row[rr][0] = 0.0;
+                    for (int j = 16; j < WB; ++j) piv[j] = row[rr][j];
                 }
             }
-            // the other lanes read what the pivot lane just wrote
             __builtin_amdgcn_wave_barrier();
-            const double rp = L.rp[grp];
-            const double p1 = piv[1];
-            // the pivot row spread over each 16-lane DPP row: lane q holds
-            // columns q, q+16, ...; a DPP broadcast hands column j to the FMAs
-            double pv[NPV];
-            pv[0] = piv[lane & 15];
-            double nl[ROWS];
-            // Gauss-Jordan: rows that were pivots earlier are eliminated too, so
-            // no back substitution is needed (rows past the system are zero)
 #pragma unroll
-            for (int rr = 0; rr < ROWS; ++rr) nl[rr] = -(row[rr][0] * rp);
-            // column k+1 first: the next step's pivot search overlaps the rest
-#pragma unroll
-            for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = fma3(nl[rr], p1, row[rr][1]);
-            candidate();
-            kmax = group_max_u32<G>(key);
-            // columns j = J0 .. J1-1 get column j+1 of the pivot row: in fours
-            // (one wait for four broadcasts), then a pair and a single
-            auto columns = [&](auto j0c, auto j1c) {
-                constexpr int J0 = decltype(j0c)::value, J1 = decltype(j1c)::value;
-                constexpr int NQ = (J1 - J0) / 4;
-                for_blocks<0, NQ>([&](auto qc) {
-                    constexpr int j = J0 + 4 * decltype(qc)::value;
-                    double p0, p1, p2, p3;
-                    row_bcast64x4<(j + 1) % 16, (j + 2) % 16, (j + 3) % 16, (j + 4) % 16>(
-                        pv[(j + 1) / 16], pv[(j + 2) / 16], pv[(j + 3) / 16], pv[(j + 4) / 16], p0, p1, p2, p3);
-#pragma unroll
-                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
-#pragma unroll
-                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
-#pragma unroll
-                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 2] = fma3(nl[rr], p2, row[rr][j + 3]);
-#pragma unroll
-                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 3] = fma3(nl[rr], p3, row[rr][j + 4]);
-                });
-                constexpr int J2 = J0 + 4 * NQ;
-                if constexpr (J1 - J2 >= 2) {
-                    constexpr int j = J2;
-                    double p0, p1;
-                    row_bcast64x2<(j + 1) % 16, (j + 2) % 16>(pv[(j + 1) / 16], pv[(j + 2) / 16], p0, p1);
-#pragma unroll
-                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], p0, row[rr][j + 1]);
-#pragma unroll
-                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j + 1] = fma3(nl[rr], p1, row[rr][j + 2]);
-                }
-                if constexpr ((J1 - J2) % 2 == 1) {
-                    constexpr int j = J1 - 1;
-                    const double pj = row_bcast64<(j + 1) % 16>(pv[(j + 1) / 16]);
-#pragma unroll
-                    for (int rr = 0; rr < ROWS; ++rr) row[rr][j] = fma3(nl[rr], pj, row[rr][j + 1]);
-                }
-            };
-            // columns up to 15 come from the first part; the rest of the row
-            // goes out only now, so the wait for the first part does not
-            // cover it.  The pivot lane's registers from 16 on are not yet
-            // updated (the update reaches register j from j+1 = 16 next).
-            constexpr int JE = WB - 1;  // columns j = 1 .. WB-2
-            constexpr int JA = JE < 15 ? JE : 15;
-            columns(std::integral_constant<int, 1>{}, std::integral_constant<int, JA>{});
-            if constexpr (NPV > 1) {
-                __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                for (int rr = 0; rr < ROWS; ++rr) {
-                    if (iam && psel == rr) {
-#pragma unroll
-                        for (int j = 16; j < WB; ++j) piv[j] = row[rr][j];
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                for (int i = 1; i < NPV; ++i) pv[i] = piv[(lane & 15) + 16 * i];
-                columns(std::integral_constant<int, JA>{}, std::integral_constant<int, JE>{});
-            }
-            // the next step's pivot lane overwrites the stale tail read here
-            __builtin_amdgcn_wave_barrier();
+            for (int i = 1; i < NPV; ++i) pv[i] = piv[(lane & 15) + 16 * i];
+            columns(std::integral_constant<int, JA>{}, std::integral_constant<int, JE>{});
         }
-    });
+    };
+    // steps K0..K1-1 in blocks of SB steps, each block a real loop at the
+    // static width of its first step
+    auto run_steps = [&](auto k0c, auto k1c, auto rfc, auto rcc) {
+        constexpr int K0 = decltype(k0c)::value, K1 = decltype(k1c)::value;
+        for_blocks<0, (K1 - K0 + T::SB - 1) / T::SB>([&](auto bc) {
+            constexpr int kb = K0 + T::SB * decltype(bc)::value;
+            const int kend = K1 < kb + T::SB ? K1 : kb + T::SB;
+#pragma nounroll
+            for (; k < kend; ++k) {
+                step(std::integral_constant<int, WIDTH - kb>{}, rfc, rcc);
+                // the next step's pivot lane overwrites the stale tail read here
+                __builtin_amdgcn_wave_barrier();
+            }
+        });
+    };
+    if constexpr (T::PFIRST) {
+        run_steps(std::integral_constant<int, 0>{}, std::integral_constant<int, NP - 1>{}, RowsPhase1{}, RowsPhase1{});
+        load_monomial_rows();
+        step(std::integral_constant<int, WIDTH - (NP - 1)>{}, RowsPhase1{}, RowsAll{});
+        __builtin_amdgcn_wave_barrier();
+        ++k;
+        run_steps(std::integral_constant<int, NP>{}, std::integral_constant<int, NS>{}, RowsAll{}, RowsAll{});
+    } else {
+        run_steps(std::integral_constant<int, 0>{}, std::integral_constant<int, NS>{}, RowsAll{}, RowsAll{});
+    }
     __builtin_amdgcn_wave_barrier();
 
     // ---- solution: after NS slides a row's registers 0..R-1 hold its
@@ -615,8 +715,8 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     uint32_t bad = 0;
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
-        const int u = kp[rr];
-        if (u < NN && live) {
+        const int u = kp[rr] - (T::PFIRST ? NP : 0);  // the w column it solved for
+        if (u >= 0 && u < NN && live) {
             const int gu = L.g[grp][u];
             int rank = 0;
 #pragma unroll
