@@ -372,21 +372,34 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     // for d = 1..n/2, which covers each pair once (twice for d = n/2, n even)
     {
         double *phi = L.u[grp] + T::PHI0;
-        const double diag = phs_r2(m, 0.0);
+        // MC > 0: the power as a constant (the usual odd powers), so the
+        // square-and-multiply folds; the branch on m is uniform
+        auto pairs = [&](auto mc) {
+            constexpr int MC = decltype(mc)::value;
+            const int mm = MC > 0 ? MC : m;
+            const double diag = phs_r2(mm, 0.0);
 #pragma unroll
-        for (int r = 0; r < NODES; ++r) {
-            const int i = t + G * r;
-            if (i < NN) {
-                phi[T::tri(i) + i] = diag;
+            for (int r = 0; r < NODES; ++r) {
+                const int i = t + G * r;
+                if (i < NN) {
+                    phi[T::tri(i) + i] = diag;
 #pragma unroll 3
-                for (int d = 1; d <= NN / 2; ++d) {
-                    int j = i + d;
-                    if (j >= NN) j -= NN;
-                    const double ddx = dxn[r] - L.u[grp][T::XI0 + 2 * j], ddy = dyn[r] - L.u[grp][T::XI0 + 2 * j + 1];
-                    const int lo = i < j ? i : j, hi = i < j ? j : i;
-                    phi[T::tri(lo) + hi] = phs_r2(m, sq_sum(ddx, ddy));
+                    for (int d = 1; d <= NN / 2; ++d) {
+                        int j = i + d;
+                        if (j >= NN) j -= NN;
+                        const double ddx = dxn[r] - L.u[grp][T::XI0 + 2 * j], ddy = dyn[r] - L.u[grp][T::XI0 + 2 * j + 1];
+                        const int lo = i < j ? i : j, hi = i < j ? j : i;
+                        phi[T::tri(lo) + hi] = phs_r2(mm, sq_sum(ddx, ddy));
+                    }
                 }
             }
+        };
+        switch (m) {
+        case 3: pairs(std::integral_constant<int, 3>{}); break;
+        case 5: pairs(std::integral_constant<int, 5>{}); break;
+        case 7: pairs(std::integral_constant<int, 7>{}); break;
+        case 9: pairs(std::integral_constant<int, 9>{}); break;
+        default: pairs(std::integral_constant<int, 0>{}); break;
         }
     }
     __builtin_amdgcn_wave_barrier();
