@@ -274,6 +274,7 @@ struct alignas(16) LuLDS {
     double u[T::S][T::USTRIDE];  // P, Phi and xi during assembly, then the pivot rows
     double rp[T::S];              // reciprocal of the current pivot
     int g[T::S][NN];
+    int rank[T::S][NN];  // rank of node i's global index inside the stencil
 };
 
 template <int NN, int NP, int R, int G, int WPB, int MINW>
@@ -351,6 +352,21 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
                 L.g[grp][i] = gid[r];
             }
         }
+        // each node's output column: the rank of its global index inside the
+        // stencil (CSR rows come out with sorted columns, as sparse() sorts)
+        __builtin_amdgcn_wave_barrier();
+        int rk[NODES];
+#pragma unroll
+        for (int r = 0; r < NODES; ++r) rk[r] = 0;
+#pragma unroll
+        for (int j = 0; j < NN; ++j) {
+            const int gj = L.g[grp][j];
+#pragma unroll
+            for (int r = 0; r < NODES; ++r) rk[r] += gj < gid[r] ? 1 : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < NODES; ++r)
+            if (t + G * r < NN) L.rank[grp][t + G * r] = rk[r];
     }
     // monomial block P (node-major) for the transposed rows; with PFIRST the
     // rows are built first and P follows from their registers
@@ -735,10 +751,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         const int u = kp[rr] - (T::PFIRST ? NP : 0);  // the w column it solved for
         if (u >= 0 && u < NN && live) {
             const int gu = L.g[grp][u];
-            int rank = 0;
-#pragma unroll
-            for (int j = 0; j < NN; ++j) rank += L.g[grp][j] < gu ? 1 : 0;
-            const long base = yk * NN + rank;
+            const long base = yk * NN + L.rank[grp][u];
             if (a.colidx) a.colidx[base] = gu;
 #pragma unroll
             for (int cc = 0; cc < R; ++cc) {
