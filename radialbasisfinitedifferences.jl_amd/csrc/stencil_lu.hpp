@@ -271,7 +271,7 @@ struct LuShape {
 template <int NN, int NP, int R, int G>
 struct alignas(16) LuLDS {
     using T = LuShape<NN, NP, R, G>;
-    double u[T::S][T::USTRIDE];  // P, Phi and xi during assembly, then the pivot rows
+    double u[T::S][T::USTRIDE];  // Phi, xi and P during assembly, then the pivot row (and P)
     double rp[T::S];              // reciprocal of the current pivot
     int g[T::S][NN];
     int rank[T::S][NN];  // rank of node i's global index inside the stencil
