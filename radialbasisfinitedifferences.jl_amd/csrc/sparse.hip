@@ -177,6 +177,70 @@ __global__ __launch_bounds__(256) void spmv_kernel(long M, const int64_t *__rest
     }
 }
 
+// SpMM with the fields split over lanes: 16 lanes per row, FL lanes share a
+// nonzero (FL divides F) and take F/FL contiguous fields each, so a nonzero's
+// x row is one contiguous run across them; the 16/FL nonzero phases are
+// summed by DPP row rotations, which leave the row's total in the phase-0
+// lanes.
+template <typename TV, int F, int FL, bool UNIFORM>
+__global__ __launch_bounds__(256) void spmm_split_kernel(long M, const int64_t *__restrict__ rowptr,
+                                                         const int *__restrict__ col, const TV *__restrict__ val,
+                                                         int n_uniform, const int *__restrict__ order,
+                                                         const double *__restrict__ x, double *__restrict__ y,
+                                                         double alpha, double beta)
+{
+    constexpr int G = 16, FPL = F / FL, NZP = G / FL;
+    static_assert(F % FL == 0 && FL <= 8, "field split");
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int sub = (int)(gid % G);
+    const int fq = sub % FL, nzp = sub / FL;
+    // rows past the end still take part in the rotations below
+    const bool live = gid / G < M;
+    const long row = !live ? M : (order ? (long)order[gid / G] : gid / G);
+    long p0 = 0, p1 = 0;
+    if (live) {
+        if (UNIFORM) { p0 = row * (long)n_uniform; p1 = p0 + n_uniform; }
+        else { p0 = rowptr[row]; p1 = rowptr[row + 1]; }
+    }
+    double acc[FPL];
+#pragma unroll
+    for (int f = 0; f < FPL; ++f) acc[f] = 0.0;
+    for (long p = p0 + nzp; p < p1; p += NZP) {
+        const double v = (double)val[p];
+        const double *xr = x + (long)col[p] * F + fq * FPL;
+#pragma unroll
+        for (int f = 0; f < FPL; ++f) acc[f] = __builtin_fma(v, xr[f], acc[f]);
+    }
+#pragma unroll
+    for (int f = 0; f < FPL; ++f) {
+        if constexpr (NZP >= 8) acc[f] += dpp_f64<0x120 | (FL)>(acc[f]);      // row_ror
+        if constexpr (NZP >= 4) acc[f] += dpp_f64<0x120 | (G / 4)>(acc[f]);
+        if constexpr (NZP >= 2) acc[f] += dpp_f64<0x120 | (G / 2)>(acc[f]);
+    }
+    if (live && nzp == 0) {
+#pragma unroll
+        for (int f = 0; f < FPL; ++f) {
+            const long o = row * F + fq * FPL + f;
+            double r = alpha * acc[f];
+            if (beta != 0.0) r = __builtin_fma(beta, y[o], r);
+            y[o] = r;
+        }
+    }
+}
+
+template <typename TV, int F, int FL>
+static void spmm_split_launch_t(long M, const int64_t *rowptr, const int *col, const TV *val, int nu,
+                                const int *order, const double *x, double *y, double alpha, double beta,
+                                hipStream_t s)
+{
+    const long threads = M * 16;
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+    if (nu > 0)
+        spmm_split_kernel<TV, F, FL, true><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, x, y, alpha, beta);
+    else
+        spmm_split_kernel<TV, F, FL, false><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, x, y, alpha, beta);
+}
+
 template <typename TV, int G, int F>
 static void spmv_launch_t(long M, const int64_t *rowptr, const int *col, const TV *val, int nu, const int *order,
                           const double *x, double *y, double alpha, double beta, hipStream_t s)
@@ -216,6 +280,29 @@ static void spmv_dispatch(long M, const int64_t *rowptr, const int *col, const T
 {
     static const int g1 = lanes_env("RBFFD_SPMV_LANES", 16);
     static const int gf = lanes_env("RBFFD_SPMM_LANES", 16);
+    // lanes sharing a nonzero: RBFFD_SPMM_FL = 1 | 2 | 4 | 8; by default 2
+    // from 4 fields up (8 fields at config 2: 189 us against 212 us with
+    // every field in every lane; 4 and 8 lanes were slower)
+    static const int fl_env = [] {
+        const char *e = std::getenv("RBFFD_SPMM_FL");
+        const int v = e ? std::atoi(e) : 0;
+        return (v == 1 || v == 2 || v == 4 || v == 8) ? v : 0;
+    }();
+    const int fl = fl_env ? fl_env : (F >= 4 ? 2 : 1);
+    if (F >= 2 && gf == 16 && fl > 1 && fl <= F) {
+        auto go = [&](auto fc, auto flc) {
+            spmm_split_launch_t<TV, decltype(fc)::value, decltype(flc)::value>(M, rowptr, col, val, nu, order, x, y,
+                                                                               alpha, beta, s);
+        };
+        using I2 = std::integral_constant<int, 2>;
+        using I4 = std::integral_constant<int, 4>;
+        using I8 = std::integral_constant<int, 8>;
+        if (F == 2) go(I2{}, I2{});
+        else if (F == 4) { if (fl == 2) go(I4{}, I2{}); else go(I4{}, I4{}); }
+        else if (F == 8) { if (fl == 2) go(I8{}, I2{}); else if (fl == 4) go(I8{}, I4{}); else go(I8{}, I8{}); }
+        else throw Error{2, "spmv: fields must be 1, 2, 4 or 8"};
+        return;
+    }
     switch (F) {
     case 1: spmv_lanes_t<TV, 1>(g1, M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
     case 2: spmv_lanes_t<TV, 2>(gf, M, rowptr, col, val, nu, order, x, y, alpha, beta, s); break;
@@ -281,15 +368,19 @@ void launch_morton_order(const double *X, long N, const double *box, uint32_t *k
 // One Shu-Osher stage: out = a u + b w + c (L w) for F fields, L in CSR with
 // G lanes per row (as spmv_kernel).  Rows with fixed[r] != 0 keep u.  out may
 // be u (each value is read and written by the same lane); out must not be w.
-template <typename TV, int G, int F, bool UNIFORM>
+template <typename TV, int G, int F, int FL, bool UNIFORM>
 __global__ __launch_bounds__(256) void rk_stage_kernel(long M, const int64_t *__restrict__ rowptr,
                                                        const int *__restrict__ col, const TV *__restrict__ val,
                                                        int n_uniform, const int *__restrict__ order,
                                                        const double *__restrict__ w, const double *u, double *out,
                                                        const uint8_t *__restrict__ fixed, double a, double b, double c)
 {
+    // FL > 1: the fields split over FL lanes per nonzero, as spmm_split_kernel
+    constexpr int FPL = F / FL, NZP = G / FL;
+    static_assert(F % FL == 0 && (FL == 1 || G == 16), "field split");
     const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const int sub = (int)(gid % G);
+    const int fq = sub % FL, nzp = sub / FL;
     const bool live = gid / G < M;
     const long row = !live ? M : (order ? (long)order[gid / G] : gid / G);
     long p0 = 0, p1 = 0;
@@ -297,23 +388,41 @@ __global__ __launch_bounds__(256) void rk_stage_kernel(long M, const int64_t *__
         if (UNIFORM) { p0 = row * (long)n_uniform; p1 = p0 + n_uniform; }
         else { p0 = rowptr[row]; p1 = rowptr[row + 1]; }
     }
-    double acc[F];
+    double acc[FPL];
 #pragma unroll
-    for (int f = 0; f < F; ++f) acc[f] = 0.0;
-    for (long p = p0 + sub; p < p1; p += G) {
+    for (int f = 0; f < FPL; ++f) acc[f] = 0.0;
+    for (long p = p0 + nzp; p < p1; p += NZP) {
         const double v = (double)val[p];
-        const long cidx = (long)col[p] * F;
+        const long cidx = (long)col[p] * F + fq * FPL;
 #pragma unroll
-        for (int f = 0; f < F; ++f) acc[f] = __builtin_fma(v, w[cidx + f], acc[f]);
+        for (int f = 0; f < FPL; ++f) acc[f] = __builtin_fma(v, w[cidx + f], acc[f]);
     }
+    if constexpr (FL == 1) {
 #pragma unroll
-    for (int f = 0; f < F; ++f) acc[f] = group_sum_f64<G>(acc[f]);
+        for (int f = 0; f < F; ++f) acc[f] = group_sum_f64<G>(acc[f]);
+    } else {
+#pragma unroll
+        for (int f = 0; f < FPL; ++f) {
+            if constexpr (NZP >= 8) acc[f] += dpp_f64<0x120 | (FL)>(acc[f]);  // row_ror
+            if constexpr (NZP >= 4) acc[f] += dpp_f64<0x120 | (G / 4)>(acc[f]);
+            if constexpr (NZP >= 2) acc[f] += dpp_f64<0x120 | (G / 2)>(acc[f]);
+        }
+    }
     if (live) {
         const bool hold = fixed && fixed[row];
+        if constexpr (FL == 1) {
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-            if (f % G == sub) {
-                const long i = row * F + f;
+            for (int f = 0; f < F; ++f) {
+                if (f % G == sub) {
+                    const long i = row * F + f;
+                    const double ui = u[i];
+                    out[i] = hold ? ui : a * ui + b * w[i] + c * acc[f];
+                }
+            }
+        } else if (nzp == 0) {
+#pragma unroll
+            for (int f = 0; f < FPL; ++f) {
+                const long i = row * F + fq * FPL + f;
                 const double ui = u[i];
                 out[i] = hold ? ui : a * ui + b * w[i] + c * acc[f];
             }
@@ -327,13 +436,15 @@ static void rk_stage_t(long M, const int64_t *rowptr, const int *col, const TV *
                        double c, hipStream_t s)
 {
     constexpr int G = 16;
+    // two lanes per nonzero from 4 fields up, as the SpMM
+    constexpr int FL = F >= 4 ? 2 : 1;
     const unsigned blocks = (unsigned)((M * G + 255) / 256);
     if (nu > 0)
-        rk_stage_kernel<TV, G, F, true><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, w, u, out, fixed, a,
-                                                               b, c);
+        rk_stage_kernel<TV, G, F, FL, true><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, w, u, out, fixed,
+                                                                   a, b, c);
     else
-        rk_stage_kernel<TV, G, F, false><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, w, u, out, fixed, a,
-                                                                b, c);
+        rk_stage_kernel<TV, G, F, FL, false><<<blocks, 256, 0, s>>>(M, rowptr, col, val, nu, order, w, u, out,
+                                                                    fixed, a, b, c);
 }
 
 template <typename TV>
