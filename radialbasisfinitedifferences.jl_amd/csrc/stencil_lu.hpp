@@ -283,6 +283,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     using T = LuShape<NN, NP, R, G>;
     constexpr int NS = T::NS, ROWS = T::ROWS, WIDTH = T::WIDTH, S = T::S, NODES = T::NODES;
     static_assert(NS <= 128, "system size above 128");
+    static_assert(!T::PFIRST || T::R1 == NODES, "RBF row slot rr holds this lane's node rr");
     __shared__ LuLDS<NN, NP, R, G> lds_all[WPB];
 
     const int lane = threadIdx.x & 63;
