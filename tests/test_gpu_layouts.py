@@ -45,7 +45,7 @@ def rowrel(got, ref):
     return (np.abs(got - ref).max(axis=-1) / np.abs(ref).max(axis=-1)).max()
 
 
-@pytest.mark.parametrize("ops", [["Dx", "Dy", "Lap"], ["Lap"]])
+@pytest.mark.parametrize("ops", [["Dx", "Dy", "Lap"], ["Lap"], ["E", "Dx", "Dy", "Dxx", "Dyy", "Dxy"]])
 @pytest.mark.parametrize("p,polydeg,n,g", CASES)
 def test_forced_layout(ctx, O, monkeypatch, p, polydeg, n, g, ops):
     import rbffd_amd
