@@ -127,6 +127,22 @@ __device__ __forceinline__ uint32_t group_max_u32(uint32_t v)
     return v;
 }
 
+// Bitwise OR over each 16-lane DPP row, in every lane of the row: with one
+// lane holding a value and the others zero, a broadcast of that lane's bits.
+__device__ __forceinline__ uint64_t row_or_u64(uint64_t b)
+{
+    uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+    lo |= dpp_u32<0xB1>(lo);   // quad_perm [1,0,3,2]
+    hi |= dpp_u32<0xB1>(hi);
+    lo |= dpp_u32<0x4E>(lo);   // quad_perm [2,3,0,1]
+    hi |= dpp_u32<0x4E>(hi);
+    lo |= dpp_u32<0x141>(lo);  // row_half_mirror
+    hi |= dpp_u32<0x141>(hi);
+    lo |= dpp_u32<0x140>(lo);  // row_mirror
+    hi |= dpp_u32<0x140>(hi);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Maximum of non-negative doubles over each aligned group of G lanes (their
 // bit patterns order like unsigned integers): DPP inside 16-lane rows, then
 // lane swaps across rows and halves, as group_max_u32.
@@ -618,7 +634,18 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         // operations in order, so those FMAs wait for the first part only.
         constexpr int W0 = WB < 16 ? WB : 16;
         constexpr int NPV = (WB + 15) / 16;
-        if (iam) L.rp[grp] = crp;
+        // 16-lane groups: the winner's reciprocal pivot and column k+1 go to
+        // the group by an OR over the DPP row (the other lanes contribute zero
+        // bits), so the next pivot search does not wait on LDS
+        uint64_t orp = 0, op1 = 0;
+        if constexpr (G == 16) {
+            if (iam) orp = (uint64_t)__double_as_longlong(crp);
+#pragma unroll
+            for (int rr = 0; rr < RF; ++rr)
+                if (iam && psel == rr) op1 = (uint64_t)__double_as_longlong(row[rr][1]);
+        } else {
+            if (iam) L.rp[grp] = crp;
+        }
 #pragma unroll
         for (int rr = 0; rr < RF; ++rr) {
             if (iam && psel == rr) {
@@ -633,12 +660,18 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         }
         // the other lanes read what the pivot lane just wrote
         __builtin_amdgcn_wave_barrier();
-        const double rp = L.rp[grp];
-        const double p1 = piv[1];
         // the pivot row spread over each 16-lane DPP row: lane q holds
         // columns q, q+16, ...; a DPP broadcast hands column j to the FMAs
         double pv[NPV];
         pv[0] = piv[lane & 15];
+        double rp, p1;
+        if constexpr (G == 16) {
+            rp = __longlong_as_double((long long)row_or_u64(orp));
+            p1 = __longlong_as_double((long long)row_or_u64(op1));
+        } else {
+            rp = L.rp[grp];
+            p1 = piv[1];
+        }
         double nl[ROWS];
         // Gauss-Jordan: rows that were pivots earlier are eliminated too, so
         // no back substitution is needed (rows past the system are zero)
