@@ -650,7 +650,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         for (int rr = 0; rr < RF; ++rr) {
             if (iam && psel == rr) {
 #pragma unroll
-                for (int j = 0; j < W0; ++j) piv[j] = row[rr][j];
+                for (int j = G == 16 ? 2 : 0; j < W0; ++j) piv[j] = row[rr][j];  // 16 lanes: 0, 1 go by DPP
                 alive[rr] = 0u;
                 kp[rr] = k;
                 pval[rr] = row[rr][0];
