@@ -281,7 +281,7 @@ static void spmv_dispatch(long M, const int64_t *rowptr, const int *col, const T
     static const int g1 = lanes_env("RBFFD_SPMV_LANES", 16);
     static const int gf = lanes_env("RBFFD_SPMM_LANES", 16);
     // lanes sharing a nonzero: RBFFD_SPMM_FL = 1 | 2 | 4 | 8; by default 2
-    // from 4 fields up (8 fields at config 2: 189 us against 212 us with
+    // from 4 fields up (8 fields over 1M nodes, n = 31: 189 us against 212 us with
     // every field in every lane; 4 and 8 lanes were slower)
     static const int fl_env = [] {
         const char *e = std::getenv("RBFFD_SPMM_FL");

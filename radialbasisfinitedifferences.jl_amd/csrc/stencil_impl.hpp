@@ -386,7 +386,7 @@ void launch_lu(const SolveArgs &a, hipStream_t s)
     constexpr int bytes = (int)sizeof(LuLDS<NN, NP, R, G>);
     // the 16-lane kernel fills a CU with 8 one-wavefront blocks as well as
     // with 4 two-wavefront ones, and single wavefronts free their slots
-    // one at a time (7.51 against 7.58 ms at config 2)
+    // one at a time (7.51 against 7.58 ms for 1M stencils of n = 31)
     constexpr int wpb = G == 16 ? 1 : lu_wpb(bytes);
     constexpr int per_block = wpb * (64 / G);
     const unsigned blocks = (unsigned)((a.ntasks + per_block - 1) / per_block);
