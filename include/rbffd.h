@@ -33,7 +33,8 @@ extern "C" {
 /* status codes (the Julia shim maps them to the exceptions the reference throws) */
 #define RBFFD_OK 0
 #define RBFFD_ERR_ARG 1          /* invalid argument          -> ArgumentError           */
-#define RBFFD_ERR_UNSUPPORTED 2  /* no kernel for this shape  -> ArgumentError           */
+#define RBFFD_ERR_UNSUPPORTED 2  /* outside the supported range (n > 128, n + P > 512)
+                                                              -> ArgumentError           */
 #define RBFFD_ERR_RUNTIME 3      /* HIP runtime failure       -> ErrorException          */
 #define RBFFD_ERR_SINGULAR 4     /* exactly singular stencil  -> LinearAlgebra.SingularException
                                     (inv at src/interpolationmatrix.jl:8)               */
@@ -56,8 +57,9 @@ extern "C" {
 
 /* precision of the local solves */
 #define RBFFD_PREC_FP64 0   /* FP64 Gauss-Jordan elimination with partial pivoting */
-#define RBFFD_PREC_MIXED 1  /* FP32 Gauss-Jordan inverse + FP64 refinement, FP64 re-solve where it
-                               does not converge */
+#define RBFFD_PREC_MIXED 1  /* experimental: FP32 Gauss-Jordan inverse + FP64 refinement, FP64 re-solve
+                               where it does not converge; built for the specialised shapes with
+                               n + P <= 64, every other shape solves in FP64 */
 #define RBFFD_PREC_AUTO 2   /* the faster of the two on this device: FP64 on MI355X, where FP32
                                issues at the FP64 rate without packed FP32 */
 
@@ -89,6 +91,8 @@ typedef struct rbffd_stats {
     int64_t refined_fallback; /* mixed precision: rows re-solved in FP64 */
     double ms_knn, ms_solve, ms_fallback;  /* device time per phase (HIP events) */
     int64_t lanes_per_stencil; /* layout of the FP64 solve kernel (chosen by the autotuner) */
+    int64_t factorizations; /* local systems factored: one per task, where a task is one output row or
+                               (Y distinct from X) the rows sharing an X node's stencil */
 } rbffd_stats;
 
 /* ---------------------------------------------------------------- context */
@@ -137,12 +141,18 @@ int rbffd_knn_device(rbffd_context *ctx, const double *X, int64_t N, const doubl
  * (y2x NULL: Y is X and y2x[k] = k).  Replaces hot loops 1 and 2 of
  * generate_operator (src/generate_operator.jl:55-166, method 3 :388-467) and
  * hyperviscosity_operator (src/hyperviscosity_operator.jl:66-161).
+ * Any stencil size 1 <= n <= 128 and polynomial degree with n + P <= 512
+ * (P = (polydeg+1)(polydeg+2)/2); shapes without a specialised kernel run a
+ * generic one.  With y2x, the rows that share an X node's stencil are solved
+ * against one factorisation of it, as the reference factors each X node once
+ * (src/generate_operator.jl:55-65,158); RBFFD_GROUP_ROWS=0 turns this off.
  * method: 1 (also method 2, same numerics) or 3.  ops/opk: nops operator kinds
  * and their derivative order k (used by XK, YK, LAPK).
  * Outputs: CSR rows of length n with sorted columns: colidx (M x n, may be
  * NULL), vals[o] (M x n, FP64) for each operator, status (M, may be NULL:
  * 0 ok, 1 singular, 2 non-finite, 3 refinement did not converge).
- * Asynchronous. */
+ * Checks stencils and y2x against N on the device first (RBFFD_ERR_ARG for an
+ * index out of range), which synchronises; the solves are then asynchronous. */
 int rbffd_weights_device(rbffd_context *ctx, const double *X, int64_t N, const double *Y, int64_t M,
                          const int32_t *stencils, int32_t n, const int32_t *y2x, int32_t p,
                          int32_t polydeg, int32_t method, int32_t nops, const int32_t *ops,

@@ -31,7 +31,7 @@ struct rbffd_buf {
 // Workspace slots of one neighbour search and the per-set buffers of a batch.
 enum ArenaSlot {
     A_PART, A_BOX, A_GRID, A_COUNTS, A_START, A_CELL_OF, A_FILL, A_CELL_XY, A_CELL_IDX, A_CELL_GRP,
-    A_SCAN_TMP, A_STENCILS, A_STATUS, A_REDO, A_NSLOT
+    A_SCAN_TMP, A_STENCILS, A_STATUS, A_REDO, A_GEN_SCRATCH, A_NSLOT
 };
 
 struct rbffd_arena {
@@ -101,7 +101,8 @@ enum Slot {
     S_CSC_PTR, S_CSC_FILL, S_CSC_PERM, S_CSC_ROW, S_H_X, S_H_Y, S_H_COL, S_H_VALS, S_H_CVALS,
     S_H_PTR64, S_H_ROW64, S_SCAN_TMP2, S_BATCH_FLAGS, S_MORTON_KEY, S_MORTON_IDX, S_CSC_PERM_IN, S_LSQ_ROWOF, S_LSQ_TCOL, S_LSQ_TPTR, S_LSQ_F64,
     S_LSQ_COUNT, S_LSQ_DENSE, S_LSQ_INFO, S_CLOSURE_T,
-    S_CLOSURE_PART, S_NSLOT
+    S_CLOSURE_PART, S_GRP_COUNTS, S_GRP_YSTART, S_GRP_FILL, S_GRP_NTASK, S_GRP_TSTART, S_GRP_YLIST, S_GRP_TASKS,
+    S_GRP_TMP, S_GEN_SCRATCH, S_NSLOT
 };
 
 struct rbffd_context {
@@ -120,8 +121,10 @@ struct rbffd_context {
     rbffd_arena lane_arena[kLanes];
     hipStream_t lane_stream[kLanes] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kLanes] = {};
-    // the last captured batch: its arguments and executable graph
+    // the last captured batch: its arguments, the workspace buffers it
+    // recorded, and its executable graph
     std::vector<int64_t> batch_key;
+    std::vector<const void *> batch_bufs;
     hipGraph_t batch_graph = nullptr;
     hipGraphExec_t batch_exec = nullptr;
     // staged device-to-host copies: two pinned buffers and host copy threads
@@ -135,7 +138,6 @@ struct rbffd_context {
 namespace {
 
 thread_local std::string g_last_error;
-
 
 template <class F>
 int guarded(F &&f)
@@ -173,6 +175,31 @@ void *grow(rbffd_buf &b, size_t bytes, bool may_alloc = true)
 }
 
 void *ws(rbffd_context *c, Slot slot, size_t bytes) { return grow(c->bufs[slot], bytes); }
+
+// Where a call takes its device buffers: the context's grow-only slots, or,
+// for the no-allocation entry point, consecutive pieces of the per-device
+// static array (which must hold them all; nothing is allocated).
+struct Workspace {
+    rbffd_context *c;
+    char *arena = nullptr;
+    size_t cap = 0, used = 0;
+    bool may_alloc = true;
+
+    void *take(size_t bytes)
+    {
+        const size_t b = ((bytes ? bytes : 16) + 255) & ~(size_t)255;
+        if (b > cap - used)
+            throw Error{RBFFD_ERR_ARG, "the call needs more device workspace than the " +
+                                           std::to_string(cap >> 20) + " MiB static array holds"};
+        void *p = arena + used;
+        used += b;
+        return p;
+    }
+    void *get(Slot slot, size_t bytes) { return arena ? take(bytes) : grow(c->bufs[slot], bytes, may_alloc); }
+    void *get(rbffd_buf &b, size_t bytes) { return arena ? take(bytes) : grow(b, bytes, may_alloc); }
+    // bytes still available (unbounded for the grow-only slots)
+    size_t room() const { return arena ? cap - used : (size_t)-1; }
+};
 
 void check_ctx(rbffd_context *c)
 {
@@ -395,26 +422,91 @@ void check_weights_args(int n, int p, int polydeg, int method, int nops, const i
 {
     if (method < 1 || method > 3) throw Error{RBFFD_ERR_ARG, "method must be 1, 2 or 3"};
     if (p < 1) throw Error{RBFFD_ERR_ARG, "PHS power must be >= 1"};
-    if (polydeg < 0 || polydeg > 8) throw Error{RBFFD_ERR_ARG, "polynomial degree must be in 0..8"};
+    if (polydeg < 0 || polydeg > 30) throw Error{RBFFD_ERR_ARG, "polynomial degree must be in 0..30"};
+    if (n < 1) throw Error{RBFFD_ERR_ARG, "stencil size must be >= 1"};
     check_ops(nops, ops, opk);
     if (precision != RBFFD_PREC_FP64 && precision != RBFFD_PREC_MIXED && precision != RBFFD_PREC_AUTO)
         throw Error{RBFFD_ERR_ARG, "unknown precision"};
-    if (precision == RBFFD_PREC_AUTO) precision = auto_precision();
     const int P = n_monomials(polydeg);
-    if (!rbffd::solve_supported(n, P, precision))
-        throw Error{RBFFD_ERR_UNSUPPORTED, "no compiled kernel for n=" + std::to_string(n) +
-                                               ", polydeg=" + std::to_string(polydeg) +
-                                               (precision ? " (mixed precision)" : "")};
+    if (!rbffd::solve_supported(n, P))
+        throw Error{RBFFD_ERR_UNSUPPORTED, "stencil size n=" + std::to_string(n) + " with polydeg=" +
+                                               std::to_string(polydeg) +
+                                               ": n must be in 1..128 and n + P at most 512"};
 }
 
+// Y rows that share an X node's stencil are solved together (one
+// factorisation per task) unless RBFFD_GROUP_ROWS=0.
+bool grouping_enabled()
+{
+    const char *env = std::getenv("RBFFD_GROUP_ROWS");
+    return !(env && env[0] == '0');
+}
+
+// Largest device-side workspace of the solves of systems above 128 equations.
+constexpr long kGenScratchBytes = 1L << 30;
+
 // Enqueue the local solves on stream s (the autotuner, when allowed, times on
-// the context stream).
-void solves_enqueue(rbffd_context *c, const double *X, const double *Y, long M, const int *stencils, int n,
-                    const int *y2x, const int *rows, long nrows, int p, int polydeg, int method, int nops,
-                    const int *ops, const int *opk, int precision, int *colidx, double *const *vals,
-                    int *status, hipStream_t s, bool may_tune)
+// the context stream).  Y rows sharing an X node's stencil (y2x given, FP64,
+// no row list) are grouped into tasks solved by one factorisation each; that
+// reads the largest group size back (a synchronisation), so a captured batch
+// (y2x NULL) never groups.  scratch: the workspace of systems above 128
+// equations (one per concurrent stream).  Returns the number of
+// factorisations (tasks) of the first operator chunk; with grouping it is
+// known on the device only, and *ntasks_dev (if not NULL) receives where.
+long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, const double *Y, long M,
+                    const int *stencils, int n, const int *y2x, const int *rows, long nrows, int p, int polydeg,
+                    int method, int nops, const int *ops, const int *opk, int precision, int *colidx,
+                    double *const *vals, int *status, hipStream_t s, bool may_tune, rbffd_buf &scratch,
+                    const int **ntasks_dev = nullptr)
 {
     const int P = n_monomials(polydeg);
+    const int cmax = std::min(rbffd::kMaxRhs, nops);
+    if (ntasks_dev) *ntasks_dev = nullptr;
+    // ---- group the Y rows by their X node
+    int ky = 1;
+    int *counts = nullptr, *fill = nullptr;
+    if (y2x && !rows && precision == RBFFD_PREC_FP64 && M > 1 && grouping_enabled()) {
+        counts = (int *)w.get(S_GRP_COUNTS, sizeof(int) * (N + 1));
+        fill = (int *)w.get(S_GRP_FILL, sizeof(int) * N);
+        int *flag = (int *)w.get(S_FLAG, sizeof(int) * 8);
+        rbffd::launch_group_count(y2x, M, N, counts, fill, s);
+        rbffd::launch_fill_i32(flag, 1, 0, s);
+        max_kernel<<<(unsigned)std::min<long>(2048, nblk(N)), 256, 0, s>>>(counts, N, flag);
+        RBFFD_HIP(hipGetLastError());
+        ky = rbffd::gen_rows_per_task(n, P, cmax, read_flag(c, flag));
+    }
+    const int *grp_x = nullptr, *grp_start = nullptr, *grp_count = nullptr, *ylist = nullptr, *ntd = nullptr;
+    if (ky > 1) {
+        int *ystart = (int *)w.get(S_GRP_YSTART, sizeof(int) * (N + 1));
+        int *ntask = (int *)w.get(S_GRP_NTASK, sizeof(int) * (N + 1));
+        int *tstart = (int *)w.get(S_GRP_TSTART, sizeof(int) * (N + 1));
+        int *yl = (int *)w.get(S_GRP_YLIST, sizeof(int) * M);
+        int *tasks = (int *)w.get(S_GRP_TASKS, sizeof(int) * 3 * M);
+        const size_t tb = rbffd::group_tmp_bytes(N);
+        void *tmp = w.get(S_GRP_TMP, tb);
+        rbffd::launch_group_build(y2x, M, N, ky, counts, ystart, fill, ntask, tstart, yl, tasks, tasks + M,
+                                  tasks + 2 * M, tmp, tb, s);
+        RBFFD_HIP(hipGetLastError());
+        grp_x = tasks;
+        grp_start = tasks + M;
+        grp_count = tasks + 2 * M;
+        ylist = yl;
+        ntd = tstart + N;
+        if (ntasks_dev) *ntasks_dev = ntd;
+    }
+    // ---- workspace of the systems above 128 equations
+    const long ntask = rows ? nrows : M;
+    const long mem_ld = rbffd::gen_mem_ld(n, P);
+    double *mem = nullptr;
+    long mem_tasks = 0;
+    if (mem_ld > 0 && ntask > 0) {
+        const long per = (long)sizeof(double) * mem_ld * (long)(n + P + cmax);
+        const long room = (long)std::min<size_t>(w.room(), (size_t)kGenScratchBytes);
+        mem_tasks = std::min(ntask, room / per);
+        if (mem_tasks < 1) mem_tasks = 1;  // w.get reports a static array too small for one system
+        mem = (double *)w.get(scratch, per * mem_tasks);
+    }
+    const bool tuned = ky == 1 && rbffd::solve_tuned(n, P, precision);
     // at most kMaxRhs right-hand sides per launch; more operators run in chunks
     for (int o0 = 0; o0 < nops; o0 += rbffd::kMaxRhs) {
         const int cnt = std::min(rbffd::kMaxRhs, nops - o0);
@@ -424,7 +516,7 @@ void solves_enqueue(rbffd_context *c, const double *X, const double *Y, long M, 
         a.stencils = stencils;
         a.y2x = y2x;
         a.rows = rows;
-        a.ntasks = rows ? nrows : M;
+        a.ntasks = ntask;
         a.n = n;
         a.P = P;
         a.m = p;
@@ -435,28 +527,56 @@ void solves_enqueue(rbffd_context *c, const double *X, const double *Y, long M, 
             a.op_k[o] = opk ? opk[o0 + o] : 0;
             a.vals[o] = vals[o0 + o];
         }
+        // right-hand-side columns of the generic kernel: row-major over
+        // (output row of the task, operator)
+        a.gen_cols = ky * cnt;
+        for (int cc = 0; cc < a.gen_cols; ++cc) {
+            const int o = cc % cnt;
+            a.col_sub[cc] = (int8_t)(cc / cnt);
+            a.col_op[cc] = (int8_t)o;
+            a.col_kind[cc] = (int8_t)a.op_kind[o];
+            a.col_k[cc] = (int8_t)a.op_k[o];
+        }
+        a.grp_x = grp_x;
+        a.grp_start = grp_start;
+        a.grp_count = grp_count;
+        a.ylist = ylist;
+        a.ntasks_dev = ntd;
+        a.scratch = mem;
+        a.mem_ld = mem_ld;
+        a.mem_tasks = mem_tasks;
         a.colidx = o0 == 0 ? colidx : nullptr;
         a.status = status;
         a.status_accumulate = o0 > 0;
         a.max_iter = 8;
         a.tol = 4e-16;
-        a.group = precision == RBFFD_PREC_FP64 ? choose_group(c, a, may_tune) : 0;
-        if (o0 == 0) c->last_group = precision == RBFFD_PREC_FP64 ? a.group : 64;
+        if (tuned && precision == RBFFD_PREC_FP64) {
+            a.group = choose_group(c, a, may_tune);
+        } else {
+            int g[rbffd::kMaxGroups];
+            rbffd::solve_groups(n, P, tuned ? precision : 0, g);
+            a.group = g[0];
+        }
+        if (o0 == 0) c->last_group = a.group;
         rbffd::launch_solve(a, precision, s);
         RBFFD_HIP(hipGetLastError());
     }
+    return ntask;
 }
 
-void weights_impl(rbffd_context *c, const double *X, long N, const double *Y, long M, const int *stencils,
+long weights_impl(rbffd_context *c, const double *X, long N, const double *Y, long M, const int *stencils,
                   int n, const int *y2x, const int *rows, long nrows, int p, int polydeg, int method,
                   int nops, const int *ops, const int *opk, int precision, int *colidx,
-                  double *const *vals, int *status)
+                  double *const *vals, int *status, const int **ntasks_dev = nullptr, Workspace *wsp = nullptr)
 {
-    (void)N;
     if (precision == RBFFD_PREC_AUTO) precision = auto_precision();
     check_weights_args(n, p, polydeg, method, nops, ops, opk, precision);
-    solves_enqueue(c, X, Y, M, stencils, n, y2x, rows, nrows, p, polydeg, method, nops, ops, opk, precision,
-                   colidx, vals, status, c->stream, true);
+    Workspace own{c};
+    Workspace &w = wsp ? *wsp : own;
+    // the static array admits no autotuning (its events are allocations)
+    return solves_enqueue(c, w, N, X, Y, M, stencils, n, y2x, rows, nrows, p, polydeg, method, nops, ops, opk,
+                          precision, colidx, vals, status, c->stream, wsp == nullptr, c->bufs[S_GEN_SCRATCH],
+                          ntasks_dev);
 }
 
 // group ids that restate calculateneighbors' masks (src/calculateneighbors.jl:16-42, 83-87)
@@ -535,9 +655,16 @@ void generate_impl(rbffd_context *c, const double *X, long N, const double *Y, l
     RBFFD_HIP(hipEventRecord(c->ev[1], s));
 
     int *status = (int *)ws(c, S_STATUS, sizeof(int) * M);
-    weights_impl(c, X, N, Y, M, st, n, y2x, nullptr, 0, p, polydeg, m3 ? 3 : 1, nops, ops, opk, precision,
-                 colidx, vals, status);
+    const int *ntasks_dev = nullptr;
+    long nfact = weights_impl(c, X, N, Y, M, st, n, y2x, nullptr, 0, p, polydeg, m3 ? 3 : 1, nops, ops, opk,
+                              precision, colidx, vals, status, &ntasks_dev);
     RBFFD_HIP(hipEventRecord(c->ev[2], s));
+    if (ntasks_dev) {
+        int h = 0;
+        RBFFD_HIP(hipMemcpyAsync(&h, ntasks_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+        RBFFD_HIP(hipStreamSynchronize(s));
+        nfact = h;
+    }
 
     int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
     long nfallback = 0;
@@ -572,6 +699,7 @@ void generate_impl(rbffd_context *c, const double *X, long N, const double *Y, l
         stats->ms_solve = t12;
         stats->ms_fallback = t23;
         stats->lanes_per_stencil = c->last_group;
+        stats->factorizations = nfact;
     }
     if (hc[1] > 0)
         throw Error{RBFFD_ERR_SINGULAR, std::to_string(hc[1]) + " stencil systems are exactly singular"};
@@ -586,6 +714,9 @@ void csc_impl(rbffd_context *c, long M, int n, long ncols, const int *colidx, in
     hipStream_t s = c->stream;
     const long nnz = M * (long)n;
     if (ncols < 1) throw Error{RBFFD_ERR_ARG, "csc: ncols must be >= 1"};
+    // column pointers, the permutation and row indices are int32 inside
+    if (nnz > INT_MAX || ncols > INT_MAX || M > INT_MAX)
+        throw Error{RBFFD_ERR_ARG, "csc: more than 2^31 - 1 entries, rows or columns"};
     if (nnz == 0) {  // sparse() of empty vectors: no columns referenced
         rbffd::launch_fill_i64(colptr, ncols + 1, base, s);
         if (ncols_used) *ncols_used = 0;
@@ -657,13 +788,13 @@ long validate_csr(rbffd_context *c, long M, const int64_t *rowptr, const int *co
     return nnz;
 }
 
-// Indices all in [0, n)
-void validate_indices(rbffd_context *c, const int *idx, long count, long n, const char *what)
+// Indices all in [0, n); flag: 2 device ints of scratch (NULL: the context's)
+void validate_indices(rbffd_context *c, const int *idx, long count, long n, const char *what, int *flag = nullptr)
 {
     if (count <= 0) return;
     hipStream_t st = c->stream;
-    int *flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
-    RBFFD_HIP(hipMemsetAsync(flag, 0, sizeof(int) * 2, st));
+    if (!flag) flag = (int *)ws(c, S_FLAG, sizeof(int) * 8);
+    rbffd::launch_fill_i32(flag, 2, 0, st);
     max_kernel<<<(unsigned)std::min<long>(2048, nblk(count)), 256, 0, st>>>(idx, count, flag);
     count_negative_kernel<<<nblk(count), 256, 0, st>>>(idx, count, flag + 1);
     int hf[2] = {0, 0};
@@ -857,8 +988,10 @@ void batch_enqueue(rbffd_context *c, const rbffd_batch_set *sets, int nsets, con
         int *status = S.status ? S.status : (int *)grow(A.b[A_STATUS], sizeof(int) * (size_t)S.N, !capture);
         int *f = flags + (long)kSetFlags * i;
         knn_enqueue(A, !capture, S.X, S.N, S.X, S.N, S.n, S.xgroup, S.qgroup, st, nullptr, f, s);
-        solves_enqueue(c, S.X, S.X, S.N, st, S.n, nullptr, nullptr, 0, S.p, S.polydeg, 1, nops, ops, opk,
-                       RBFFD_PREC_FP64, S.colidx, S.vals, status, s, !capture);
+        Workspace w{c};
+        w.may_alloc = !capture;
+        solves_enqueue(c, w, S.N, S.X, S.X, S.N, st, S.n, nullptr, nullptr, 0, S.p, S.polydeg, 1, nops, ops, opk,
+                       RBFFD_PREC_FP64, S.colidx, S.vals, status, s, !capture, A.b[A_GEN_SCRATCH]);
         status_count_kernel<<<nblk(S.N), 256, 0, s>>>(status, S.N, f + 2);
         RBFFD_HIP(hipGetLastError());
     }
@@ -868,6 +1001,19 @@ void batch_enqueue(rbffd_context *c, const rbffd_batch_set *sets, int nsets, con
             RBFFD_HIP(hipStreamWaitEvent(main, c->join_ev[l], 0));
         }
     }
+}
+
+// Every context-owned buffer a captured batch can reference.  A buffer that
+// grows is freed and reallocated, so a graph recorded before that would
+// replay into freed memory: batch_impl replays only while this list is
+// unchanged.
+std::vector<const void *> batch_buffers(const rbffd_context *c)
+{
+    std::vector<const void *> v;
+    for (const auto &A : c->lane_arena)
+        for (const auto &b : A.b) v.push_back(b.p);
+    for (const auto &b : c->bufs) v.push_back(b.p);
+    return v;
 }
 
 std::vector<int64_t> batch_key(const rbffd_batch_set *sets, int nsets, int nops, const int *ops, const int *opk)
@@ -923,7 +1069,7 @@ void batch_impl(rbffd_context *c, int nsets, const rbffd_batch_set *sets, int no
     hipStream_t main = c->stream;
     auto key = batch_key(sets, nsets, nops, ops, opk);
     RBFFD_HIP(hipEventRecord(c->ev[0], main));
-    if (use_graph && c->batch_exec && key == c->batch_key) {
+    if (use_graph && c->batch_exec && key == c->batch_key && batch_buffers(c) == c->batch_bufs) {
         RBFFD_HIP(hipGraphLaunch(c->batch_exec, main));
     } else {
         batch_enqueue(c, sets, nsets, lane_of, nops, ops, opk, flags, false);  // also tunes the layouts
@@ -931,6 +1077,7 @@ void batch_impl(rbffd_context *c, int nsets, const rbffd_batch_set *sets, int no
             if (c->batch_exec) { RBFFD_HIP(hipGraphExecDestroy(c->batch_exec)); c->batch_exec = nullptr; }
             if (c->batch_graph) { RBFFD_HIP(hipGraphDestroy(c->batch_graph)); c->batch_graph = nullptr; }
             c->batch_key.clear();
+            c->batch_bufs.clear();
             for (int l = 0; l < kLanes; ++l) {
                 if (!c->lane_stream[l]) RBFFD_HIP(hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking));
                 if (!c->join_ev[l]) RBFFD_HIP(hipEventCreateWithFlags(&c->join_ev[l], hipEventDisableTiming));
@@ -951,6 +1098,7 @@ void batch_impl(rbffd_context *c, int nsets, const rbffd_batch_set *sets, int no
             c->batch_graph = g;
             RBFFD_HIP(hipGraphInstantiate(&c->batch_exec, g, nullptr, nullptr, 0));
             c->batch_key = key;
+            c->batch_bufs = batch_buffers(c);
         }
     }
     RBFFD_HIP(hipEventRecord(c->ev[1], main));
@@ -1173,8 +1321,15 @@ int rbffd_weights_device(rbffd_context *c, const double *X, int64_t N, const dou
     return guarded([&] {
         check_ctx(c);
         if (!X || !Y || !stencils || !ops || !vals) throw Error{RBFFD_ERR_ARG, "null pointer"};
+        if (N < 1 || M < 0 || N > INT_MAX || M > INT_MAX) throw Error{RBFFD_ERR_ARG, "weights: bad sizes"};
         for (int o = 0; o < nops && o < RBFFD_MAX_OPS; ++o)
             if (!vals[o]) throw Error{RBFFD_ERR_ARG, "null value array"};
+        if (M == 0) return;
+        if (n >= 1 && (long)M * n > INT_MAX) throw Error{RBFFD_ERR_ARG, "weights: more than 2^31 - 1 entries"};
+        // every index the kernels follow stays inside X
+        const long nst = (y2x ? N : M) * (long)n;
+        if (n >= 1) validate_indices(c, stencils, nst, N, "weights stencils");
+        if (y2x) validate_indices(c, y2x, M, N, "weights y2x");
         weights_impl(c, X, N, Y, M, stencils, n, y2x, nullptr, 0, p, polydeg, method, nops, ops, opk, precision,
                      colidx, vals, status);
     });
@@ -1251,6 +1406,8 @@ int rbffd_generate_operator(rbffd_context *c, const double *X, int64_t N, const 
         const bool m3 = (Y == nullptr);
         const long Mr = m3 ? N : M;
         const long nnz = Mr * (long)n;
+        if (nnz > INT_MAX || N > INT_MAX || Mr > INT_MAX)
+            throw Error{RBFFD_ERR_ARG, "generate_operator: more than 2^31 - 1 entries or nodes"};
         hipStream_t s = c->stream;
         double *dX = (double *)ws(c, S_H_X, sizeof(double) * 2 * N);
         RBFFD_HIP(hipMemcpyAsync(dX, X, sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));

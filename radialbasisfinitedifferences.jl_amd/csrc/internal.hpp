@@ -9,6 +9,7 @@ namespace rbffd {
 
 constexpr int kMaxOps = 10;
 constexpr int kMaxRhs = 6;   // right-hand sides per solve launch
+constexpr int kMaxGenRhs = 72;  // right-hand-side columns of the generic kernel (rows x operators)
 
 // Failures inside the library are thrown as this and turned into a status code
 // plus a thread-local message at the C-ABI boundary.
@@ -78,17 +79,51 @@ struct SolveArgs {
     int max_iter;
     double tol;
     int group;              // lanes per stencil of the FP64 kernel (0: default)
+    // ---- the generic kernel (stencil_gen.hpp): any (n, P), and output rows
+    // sharing one stencil.  Tasks of several rows: task -> X node grp_x, its
+    // rows ylist[grp_start .. grp_start + grp_count); NULL: one row per task.
+    const int *grp_x, *grp_start, *grp_count, *ylist;
+    const int *ntasks_dev;  // if set, tasks at or past *ntasks_dev do nothing
+    int gen_cols;           // right-hand-side columns of the widest task
+    // right-hand-side column cc: output row col_sub[cc] of the task, operator
+    // col_op[cc] (index into vals) of kind col_kind[cc] and order col_k[cc]
+    int8_t col_sub[kMaxGenRhs], col_op[kMaxGenRhs], col_kind[kMaxGenRhs], col_k[kMaxGenRhs];
+    // systems above 128 equations: the augmented systems of mem_tasks stencils
+    // at a time, mem_ld rows (padded) by n + P + nops columns each
+    double *scratch;
+    long mem_ld, mem_tasks;
 };
 
-// Which (n, P) shapes have compiled kernels.
-bool solve_supported(int n, int P, int precision);
-// precision: 0 = FP64 Gauss-Jordan, 1 = FP32 Gauss-Jordan inverse + FP64 refinement.
-// a.nops <= kMaxRhs.
+// Shapes the library solves: stencils of 1..128 nodes (the neighbour
+// search's limit) and systems of at most 512 equations.
+bool solve_supported(int n, int P);
+// Shapes with a specialised kernel (precision 1: with a mixed-precision one).
+bool solve_tuned(int n, int P, int precision);
+// precision: 0 = FP64 Gauss-Jordan, 1 = FP32 Gauss-Jordan inverse + FP64
+// refinement (specialised shapes only; the others solve in FP64).  Tasks of
+// several rows (a.grp_x) and shapes without a specialised kernel run the
+// generic kernel.  a.nops <= kMaxRhs.
 void launch_solve(const SolveArgs &a, int precision, hipStream_t s);
 // Compiled lane-group sizes for a shape: groups[0] default, groups[1..kMaxGroups-1]
 // alternatives or 0.
 constexpr int kMaxGroups = 3;
 void solve_groups(int n, int P, int precision, int *groups);
+// Output rows per task of the generic kernel for nops operators when an X node
+// has at most maxrows of them (1: no grouping).
+int gen_rows_per_task(int n, int P, int nops, int maxrows);
+// Workspace doubles per stencil of the generic kernel for systems above 128
+// equations (0 below): mem_ld rows by n + P + nops columns.
+long gen_mem_ld(int n, int P);
+// Y rows grouped by the X node of their stencil, in tasks of at most ky rows.
+// launch_group_count: counts[x] = rows with y2x = x (counts: N + 1, fill: N,
+// both cleared here).  launch_group_build: ystart, ntask, tstart (N + 1 each),
+// ylist (M), grp_x, grp_start, grp_count (M each; the task count, tstart[N],
+// is at most M).
+size_t group_tmp_bytes(long N);
+void launch_group_count(const int *y2x, long M, long N, int *counts, int *fill, hipStream_t s);
+void launch_group_build(const int *y2x, long M, long N, int ky, const int *counts, int *ystart, int *fill,
+                        int *ntask, int *tstart, int *ylist, int *grp_x, int *grp_start, int *grp_count, void *tmp,
+                        size_t tmp_bytes, hipStream_t s);
 
 // Compiled stencil shapes (n, P): the reference's tests (20,10) and (42,21),
 // and the synthetic node-set shapes (13,6), (21,10), (31,15), (43,21), (57,28).
@@ -98,6 +133,16 @@ void solve_groups(int n, int P, int precision, int *groups);
     void groups_##nn##_##pp(int precision, int *groups);
 RBFFD_SHAPES(RBFFD_DECLARE_SHAPE)
 #undef RBFFD_DECLARE_SHAPE
+// Generic kernel per padded system size NSB (R: right-hand-side columns
+// needed): gen_lanes(NSB) lanes per stencil, right-hand-side blocks of 3, 6 and
+// gen_rb(NSB) columns (several output rows per task where that exceeds 6).
+#define RBFFD_GEN_BUCKETS(X) X(16) X(32) X(64) X(96) X(128)
+constexpr int gen_lanes(int nsb) { return nsb <= 16 ? 16 : (nsb <= 32 ? 32 : 64); }
+constexpr int gen_rb(int nsb) { return nsb <= 32 ? 48 : (nsb <= 64 ? 72 : 6); }
+#define RBFFD_DECLARE_GEN(nsb) void launch_gen_##nsb(const SolveArgs &a, int R, hipStream_t s);
+RBFFD_GEN_BUCKETS(RBFFD_DECLARE_GEN)
+#undef RBFFD_DECLARE_GEN
+void launch_gen_mem(const SolveArgs &a, hipStream_t s);
 
 // --------------------------------------------------- sparse (K3, K4)
 void launch_fill_rowptr(int64_t *rowptr, long M, int n, hipStream_t s);

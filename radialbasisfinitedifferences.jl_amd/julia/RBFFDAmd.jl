@@ -10,6 +10,8 @@
 #   hyperviscosity_operator(k, X, p, n, polydeg)                 src/hyperviscosity_operator.jl:314
 #
 #   solve(D, f)  (= D \ f of test/poisson_test.jl:121)           least squares on the GPU
+#   DeviceOperator(A); mul!(y, op, x); op * x                    A * x of test/poisson_test.jl:121,
+#                                                                examples/adv_diff_test.jl:151-152
 #
 # Each generator returns SparseMatrixCSC{Float64,Int64} in the reference's layout:
 # 1-based, rows sorted inside each column, explicit zeros kept, dims
@@ -25,7 +27,7 @@ using LinearAlgebra
 using SparseArrays
 using StaticArrays
 
-export generate_operator, hyperviscosity_operator, solve
+export generate_operator, hyperviscosity_operator, solve, DeviceOperator
 
 const LIB = get(ENV, "RBFFD_LIB", joinpath(@__DIR__, "..", "rbffd_amd", "librbffd.so"))
 
@@ -55,6 +57,7 @@ struct Stats
     ms_solve::Float64
     ms_fallback::Float64
     lanes_per_stencil::Int64
+    factorizations::Int64
 end
 
 const CONTEXTS = Dict{Int,Ptr{Cvoid}}()
@@ -130,7 +133,10 @@ function _operators(X::AbstractVector{<:SVector{2,Float64}}, Y, p, n, polydeg,
     check(code)
     nc = ncols[]
     cp = colptr[1:nc+1]
-    return Tuple(SparseMatrixCSC(M, nc, cp, rowval, v) for v in nz)
+    # every matrix owns its colptr and rowval, as the reference's separate
+    # sparse() calls return independent matrices (src/generate_operator.jl:176-181)
+    return Tuple(SparseMatrixCSC(M, nc, i == 1 ? cp : copy(cp), i == 1 ? rowval : copy(rowval), v)
+                 for (i, v) in enumerate(nz))
 end
 
 const SIX = Int32[OP_E, OP_DX, OP_DY, OP_DXX, OP_DYY, OP_DXY]
@@ -169,6 +175,58 @@ function _upload(c, a::Vector)
                                c, p, pointer(a), sizeof(a)))
     return p
 end
+
+# ---- A * x of the harnesses (test/poisson_test.jl:121, examples/adv_diff_test.jl:151-152)
+# on the device: the CSR of A (the CSC of transpose(A)) is uploaded once and
+# applied by rbffd_spmv_device; x and y cross PCIe per product.  Fields: a
+# vector, or an N x F matrix with F in 1, 2, 4, 8 (node-major on the device).
+mutable struct DeviceOperator
+    m::Int
+    n::Int
+    ctx::Ptr{Cvoid}
+    rowptr::Ptr{Cvoid}   # Int64, 0-based
+    col::Ptr{Cvoid}      # Int32, 0-based
+    vals::Ptr{Cvoid}     # Float64
+    function DeviceOperator(A::SparseMatrixCSC{Float64})
+        M, N = size(A)
+        At = sparse(transpose(A))              # its colptr/rowval are the CSR of A
+        c = context()
+        rp = _upload(c, Int64.(At.colptr .- 1))
+        cl = _upload(c, Int32.(At.rowval .- 1))
+        vl = _upload(c, Vector{Float64}(At.nzval))
+        op = new(M, N, c, rp, cl, vl)
+        finalizer(o -> foreach(p -> _dfree(o.ctx, p), (o.rowptr, o.col, o.vals)), op)
+        return op
+    end
+end
+
+Base.size(op::DeviceOperator) = (op.m, op.n)
+
+function LinearAlgebra.mul!(y::AbstractVecOrMat{Float64}, op::DeviceOperator, x::AbstractVecOrMat{Float64})
+    F = size(x, 2)
+    size(x, 1) == op.n && size(y, 1) == op.m && size(y, 2) == F || throw(DimensionMismatch("mul!"))
+    xr = Matrix{Float64}(transpose(reshape(x, op.n, F)))  # node-major: x[f, node]
+    yr = Matrix{Float64}(undef, F, op.m)
+    c = op.ctx
+    xd = _upload(c, vec(xr))
+    yd = _dalloc(c, sizeof(yr))
+    try
+        check(ccall((:rbffd_spmv_device, LIB), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, Ptr{Cvoid}, Ptr{Cvoid},
+                     Float64, Float64, Int32, Ptr{Cvoid}),
+                    c, op.m, op.rowptr, op.col, op.vals, Int32(0), Int32(F), xd, yd, 1.0, 0.0, Int32(0), C_NULL))
+        GC.@preserve yr check(ccall((:rbffd_memcpy_d2h, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                                    c, pointer(yr), yd, sizeof(yr)))
+    finally
+        _dfree(c, xd)
+        _dfree(c, yd)
+    end
+    y .= reshape(transpose(yr), size(y))
+    return y
+end
+
+Base.:*(op::DeviceOperator, x::AbstractVector{Float64}) = mul!(Vector{Float64}(undef, op.m), op, x)
+Base.:*(op::DeviceOperator, x::AbstractMatrix{Float64}) = mul!(Matrix{Float64}(undef, op.m, size(x, 2)), op, x)
 
 function solve(D::SparseMatrixCSC{Float64}, f::AbstractVector{Float64}; method::Symbol = :auto,
                tol::Float64 = 1e-12, max_iters::Integer = 100_000)

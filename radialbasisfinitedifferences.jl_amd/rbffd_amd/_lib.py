@@ -64,6 +64,7 @@ class rbffd_stats(ctypes.Structure):
         ("ms_solve", ctypes.c_double),
         ("ms_fallback", ctypes.c_double),
         ("lanes_per_stencil", ctypes.c_int64),
+        ("factorizations", ctypes.c_int64),
     ]
 
     def as_dict(self):

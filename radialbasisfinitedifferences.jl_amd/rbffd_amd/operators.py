@@ -124,7 +124,12 @@ def operators(X, Y, p, n, polydeg, ops, k=0, boundary=None, precision="fp64", ct
     t_call = time.perf_counter() - t0
     check(code)
     nc = int(ncols.value)
-    mats = tuple(sp.csc_matrix((nz[o], rowval, colptr[: nc + 1]), shape=(M, nc)) for o in range(nops))
+    # every matrix owns its index arrays, as the reference's separate sparse()
+    # calls return independent matrices (src/generate_operator.jl:176-181): a
+    # structural edit of one leaves the others intact
+    cp = colptr[: nc + 1]
+    mats = tuple(sp.csc_matrix((nz[o], rowval if o == 0 else rowval.copy(), cp if o == 0 else cp.copy()),
+                               shape=(M, nc), copy=False) for o in range(nops))
     if return_stats:
         st = stats.as_dict()
         st["host_call_s"] = t_call  # the C entry point alone: H2D, kernels, D2H
@@ -359,6 +364,43 @@ class Engine:
         order = self.ctx.empty((X.shape[0],), np.int32)
         check(self.L.rbffd_locality_order_device(self.ctx.handle, X.ptr, X.shape[0], order.ptr))
         return order
+
+
+class DeviceOperator:
+    """A sparse operator resident in device memory, applied by librbffd's SpMV
+    (rbffd_spmv_device): the `A * x` / `mul!(y, A, x)` of the reference's
+    harnesses (test/poisson_test.jl:121, examples/adv_diff_test.jl:151-152) on
+    a matrix that generate_operator returned (or any scipy sparse matrix).
+    The CSR of A (int64 row pointers, int32 columns, float64 values) is
+    uploaded once; products take host or device vectors, with 1 to 8 fields
+    stored node-major (x[node, field])."""
+
+    def __init__(self, A, ctx=None):
+        self.ctx = ctx or default_context()
+        self.eng = Engine(self.ctx)
+        R = sp.csr_matrix(A)
+        R.sort_indices()
+        if R.nnz > np.iinfo(np.int32).max or R.shape[1] > np.iinfo(np.int32).max:
+            raise ValueError("DeviceOperator: more than 2^31 - 1 entries or columns")
+        self.shape = R.shape
+        self.nnz = R.nnz
+        self.rowptr = self.ctx.to_device(R.indptr.astype(np.int64))
+        self.col = self.ctx.to_device(R.indices.astype(np.int32))
+        self.vals = self.ctx.to_device(R.data.astype(np.float64))
+
+    def apply(self, x: DeviceArray, y: DeviceArray, alpha=1.0, beta=0.0):
+        """y = alpha A x + beta y on device arrays (asynchronous on the context stream)."""
+        if x.shape[0] != self.shape[1] or y.shape[0] != self.shape[0] or x.shape[1:] != y.shape[1:]:
+            raise ValueError("DeviceOperator: shape mismatch")
+        return self.eng.spmv(self.col, self.vals, x, y, rowptr=self.rowptr, alpha=alpha, beta=beta)
+
+    def __matmul__(self, x):
+        """A @ x for a host vector or (N, F) block; returns a host array."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        xd = self.ctx.to_device(x)
+        yd = self.ctx.empty((self.shape[0],) + x.shape[1:], np.float64)
+        self.apply(xd, yd)
+        return yd.get()
 
 
 def solve(D, f, method="auto", tol=1e-12, max_iters=100000, ctx=None, info=False):

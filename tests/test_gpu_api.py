@@ -140,7 +140,7 @@ def test_error_behaviour(ctx):
     with pytest.raises(ValueError):
         rbffd_amd.generate_operator(X[:10], X[:10], 5, 13, 2, ctx=ctx)       # n > N
     with pytest.raises(NotImplementedError):
-        rbffd_amd.generate_operator(X, X, 5, 14, 2, ctx=ctx)                 # no kernel for this shape
+        rbffd_amd.generate_operator(X, X, 5, 129, 2, ctx=ctx)                # n above the supported 128
     Xd = np.concatenate([X, X[50:51]])                                     # duplicated node
     with pytest.raises(rbffd_amd.SingularException):
         rbffd_amd.generate_operator(Xd, Xd, 5, 13, 2, ctx=ctx)

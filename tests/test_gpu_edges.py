@@ -98,6 +98,16 @@ def test_host_entry_point_one_based(ctx, O):
                                      None)
     _lib.check(code)
     assert ncols.value == N
-    ref = O.generate_operator(X, X, 5, n, 2, ops=("Dx", "Lap"))
+    ref = O.generate_operator(X, X, 5, n, 2, ops=("Dx", "Lap"), quad=True)
     cp, rv, _, _ = ref["Dx"]
     assert colptr[0] == 1 and np.array_equal(colptr - 1, cp) and np.array_equal(rowval - 1, rv)
+    # the values through the host entry point, per-row relative error vs the quad oracle
+    for o, name in enumerate(("Dx", "Lap")):
+        _, _, zref, _ = ref[name]
+        num = np.zeros(N)
+        den = np.zeros(N)
+        np.maximum.at(num, rv, np.abs(nz[o] - zref))
+        np.maximum.at(den, rv, np.abs(zref))
+        err = (num / den).max()
+        print(f"  {name}: nzval vs quad {err:.2e}", flush=True)
+        assert err <= 1e-10
