@@ -159,6 +159,24 @@ int rbffd_weights_device(rbffd_context *ctx, const double *X, int64_t N, const d
                          const int32_t *opk, int32_t precision, int32_t *colidx, double *const *vals,
                          int32_t *status);
 
+/* rbffd_weights_device without allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as rbffd_weights_device, except that every
+ * device buffer the call needs beyond its inputs and outputs (index checks,
+ * the grouping of Y rows by stencil, the workspace of systems above 128
+ * equations) comes from one fixed 64 MiB array in each GPU's memory, held
+ * until the process ends; no layout autotuning runs.  RBFFD_ERR_ARG when a
+ * call needs more than the array holds (grouping takes about 20 bytes per X
+ * node and 16 per Y row).  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (two contexts, two threads) race.  Synchronises for
+ * the index checks, like rbffd_weights_device. */
+int rbffd_weights_device_noalloc(rbffd_context *ctx, const double *X, int64_t N, const double *Y, int64_t M,
+                                 const int32_t *stencils, int32_t n, const int32_t *y2x, int32_t p,
+                                 int32_t polydeg, int32_t method, int32_t nops, const int32_t *ops,
+                                 const int32_t *opk, int32_t precision, int32_t *colidx, double *const *vals,
+                                 int32_t *status);
+
 /* --------------------------------------------- whole generation pipeline
  * kNN + nearest-X of every Y + weights, i.e. generate_operator /
  * hyperviscosity_operator up to (not including) the CSC assembly.

@@ -133,11 +133,41 @@ struct rbffd_context {
     std::unique_ptr<rbffd_host_pool> pool;
     // rocSOLVER handle of the direct sparse solve (created on first use)
     void *solver = nullptr;
+    // this device's copy of the static workspace of the no-allocation entry point
+    char *noalloc = nullptr;
 };
+
+// Device workspace of rbffd_weights_device_noalloc: a fixed array in every
+// GPU's memory, held until the process ends.  Host code never passes this
+// name to a kernel or a copy (on the host it only stands for the symbol): it
+// uses the device address hipGetSymbolAddress returns for the current device,
+// which rbffd_create looks up once per device.
+constexpr size_t kNoAllocBytes = 64u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 namespace {
 
 thread_local std::string g_last_error;
+
+// scratch's device address, per device (looked up once, under the lock)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+char *scratch_address(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Error{RBFFD_ERR_RUNTIME, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        g_scratch_ptr[device] = (char *)ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
 
 template <class F>
 int guarded(F &&f)
@@ -1153,10 +1183,24 @@ int rbffd_create(int device, rbffd_context **out)
         int count = 0;
         RBFFD_HIP(hipGetDeviceCount(&count));
         if (device < 0 || device >= count) throw Error{RBFFD_ERR_ARG, "no such device"};
+        RBFFD_HIP(hipSetDevice(device));
+        // the no-allocation entry point's array, looked up with the device
+        // current; without it that entry point reports an error and the rest
+        // of the library works as before
+        char *sp = nullptr;
+        try {
+            sp = scratch_address(device);
+        } catch (const Error &) {
+            sp = nullptr;
+        }
         auto *c = new rbffd_context();
         c->device = device;
-        RBFFD_HIP(hipSetDevice(device));
-        RBFFD_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+        c->noalloc = sp;
+        const hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw Error{RBFFD_ERR_RUNTIME, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e)};
+        }
         c->stream = c->own_stream;
         *out = c;
     });
@@ -1332,6 +1376,32 @@ int rbffd_weights_device(rbffd_context *c, const double *X, int64_t N, const dou
         if (y2x) validate_indices(c, y2x, M, N, "weights y2x");
         weights_impl(c, X, N, Y, M, stencils, n, y2x, nullptr, 0, p, polydeg, method, nops, ops, opk, precision,
                      colidx, vals, status);
+    });
+}
+
+int rbffd_weights_device_noalloc(rbffd_context *c, const double *X, int64_t N, const double *Y, int64_t M,
+                                 const int32_t *stencils, int32_t n, const int32_t *y2x, int32_t p,
+                                 int32_t polydeg, int32_t method, int32_t nops, const int32_t *ops,
+                                 const int32_t *opk, int32_t precision, int32_t *colidx, double *const *vals,
+                                 int32_t *status)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!X || !Y || !stencils || !ops || !vals) throw Error{RBFFD_ERR_ARG, "null pointer"};
+        if (N < 1 || M < 0 || N > INT_MAX || M > INT_MAX) throw Error{RBFFD_ERR_ARG, "weights: bad sizes"};
+        for (int o = 0; o < nops && o < RBFFD_MAX_OPS; ++o)
+            if (!vals[o]) throw Error{RBFFD_ERR_ARG, "null value array"};
+        if (!c->noalloc) throw Error{RBFFD_ERR_RUNTIME, "no static workspace on this device"};
+        if (M == 0) return;
+        if (n >= 1 && (long)M * n > INT_MAX) throw Error{RBFFD_ERR_ARG, "weights: more than 2^31 - 1 entries"};
+        // every buffer below comes from the static array; nothing is allocated
+        Workspace w{c, c->noalloc, kNoAllocBytes, 0, false};
+        int *flag = (int *)w.take(sizeof(int) * 8);
+        const long nst = (y2x ? N : M) * (long)n;
+        if (n >= 1) validate_indices(c, stencils, nst, N, "weights stencils", flag);
+        if (y2x) validate_indices(c, y2x, M, N, "weights y2x", flag);
+        weights_impl(c, X, N, Y, M, stencils, n, y2x, nullptr, 0, p, polydeg, method, nops, ops, opk, precision,
+                     colidx, vals, status, nullptr, &w);
     });
 }
 

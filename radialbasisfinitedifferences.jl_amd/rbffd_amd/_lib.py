@@ -116,6 +116,8 @@ SIGNATURES = {
     "rbffd_knn_device": (ctypes.c_int, [_V, _V, _I64, _V, _I64, _I32, _V, _V, _V, _V]),
     "rbffd_weights_device": (ctypes.c_int, [_V, _V, _I64, _V, _I64, _V, _I32, _V, _I32, _I32, _I32, _I32,
                                             _V, _V, _I32, _V, _V, _V]),
+    "rbffd_weights_device_noalloc": (ctypes.c_int, [_V, _V, _I64, _V, _I64, _V, _I32, _V, _I32, _I32, _I32, _I32,
+                                                    _V, _V, _I32, _V, _V, _V]),
     "rbffd_generate_device": (ctypes.c_int, [_V, _V, _I64, _V, _I64, _I32, _I32, _I32,
                                              ctypes.POINTER(rbffd_boundary), _I32, _V, _V, _I32, _V, _V, _V, _V,
                                              ctypes.POINTER(rbffd_stats)]),
