@@ -216,6 +216,13 @@ int rbffd_spmv_device(rbffd_context *ctx, int64_t M, const int64_t *rowptr, cons
                       const void *vals, int32_t vals_type, int32_t F, const double *x, double *y,
                       double alpha, double beta, int32_t uniform_n, const int32_t *row_order);
 
+/* *out = the sum of x[0 .. n) (device pointers, x 16-byte aligned): a
+ * read-only stream at the device's HBM read rate, the ceiling SpMV (whose
+ * traffic is almost all reads) is measured against.  No reference
+ * counterpart: a measurement aid.  The last bits of the sum depend on the
+ * order of the blocks' atomic adds.  Asynchronous. */
+int rbffd_read_sum_device(rbffd_context *ctx, const double *x, int64_t n, double *out);
+
 /* A locality order of N nodes (Morton order of their coordinates): with it as
  * row_order, the rows of one workgroup are a compact patch of the domain and
  * share most of the x values they gather.  Results are identical with or

@@ -115,6 +115,8 @@ struct rbffd_context {
     hipEvent_t tune_ev[2] = {};
     // autotuner: (n, P, right-hand sides) -> lanes per stencil of the FP64 kernel
     std::map<std::tuple<int, int, int>, int> lu_group;
+    // (n, P, right-hand sides) -> whether Y rows sharing a stencil solve grouped
+    std::map<std::tuple<int, int, int>, bool> group_rows;
     int last_group = 0;  // lanes per stencil of the last solve launch
     // neighbour-search workspace of the single-set calls, and one per batch branch
     rbffd_arena knn_arena;
@@ -474,6 +476,8 @@ bool grouping_enabled()
 
 // Largest device-side workspace of the solves of systems above 128 equations.
 constexpr long kGenScratchBytes = 1L << 30;
+// Rows from which the choice between grouped and per-row solves is timed.
+constexpr long kGroupTuneRows = 1L << 16;
 
 // Enqueue the local solves on stream s (the autotuner, when allowed, times on
 // the context stream).  Y rows sharing an X node's stencil (y2x given, FP64,
@@ -503,7 +507,7 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
         rbffd::launch_fill_i32(flag, 1, 0, s);
         max_kernel<<<(unsigned)std::min<long>(2048, nblk(N)), 256, 0, s>>>(counts, N, flag);
         RBFFD_HIP(hipGetLastError());
-        ky = rbffd::gen_rows_per_task(n, P, cmax, read_flag(c, flag));
+        ky = rbffd::gen_rows_per_task(n, P, cmax, read_flag(c, flag), (double)M / (double)N);
     }
     const int *grp_x = nullptr, *grp_start = nullptr, *grp_count = nullptr, *ylist = nullptr, *ntd = nullptr;
     if (ky > 1) {
@@ -536,61 +540,105 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
         if (mem_tasks < 1) mem_tasks = 1;  // w.get reports a static array too small for one system
         mem = (double *)w.get(scratch, per * mem_tasks);
     }
-    const bool tuned = ky == 1 && rbffd::solve_tuned(n, P, precision);
-    // at most kMaxRhs right-hand sides per launch; more operators run in chunks
-    for (int o0 = 0; o0 < nops; o0 += rbffd::kMaxRhs) {
-        const int cnt = std::min(rbffd::kMaxRhs, nops - o0);
-        rbffd::SolveArgs a{};
-        a.X = X;
-        a.Y = Y;
-        a.stencils = stencils;
-        a.y2x = y2x;
-        a.rows = rows;
-        a.ntasks = ntask;
-        a.n = n;
-        a.P = P;
-        a.m = p;
-        a.method = method == 3 ? 3 : 1;
-        a.nops = cnt;
-        for (int o = 0; o < cnt; ++o) {
-            a.op_kind[o] = ops[o0 + o];
-            a.op_k[o] = opk ? opk[o0 + o] : 0;
-            a.vals[o] = vals[o0 + o];
+    // every operator chunk, with tasks of kyv rows (1: one row per task)
+    auto launch_all = [&](int kyv) {
+        const bool grouped = kyv > 1;
+        const bool tuned = !grouped && rbffd::solve_tuned(n, P, precision);
+        // at most kMaxRhs right-hand sides per launch; more operators run in chunks
+        for (int o0 = 0; o0 < nops; o0 += rbffd::kMaxRhs) {
+            const int cnt = std::min(rbffd::kMaxRhs, nops - o0);
+            rbffd::SolveArgs a{};
+            a.X = X;
+            a.Y = Y;
+            a.stencils = stencils;
+            a.y2x = y2x;
+            a.rows = rows;
+            a.ntasks = ntask;
+            a.n = n;
+            a.P = P;
+            a.m = p;
+            a.method = method == 3 ? 3 : 1;
+            a.nops = cnt;
+            for (int o = 0; o < cnt; ++o) {
+                a.op_kind[o] = ops[o0 + o];
+                a.op_k[o] = opk ? opk[o0 + o] : 0;
+                a.vals[o] = vals[o0 + o];
+            }
+            // right-hand-side columns of the generic kernel: row-major over
+            // (output row of the task, operator)
+            a.gen_cols = kyv * cnt;
+            for (int cc = 0; cc < a.gen_cols; ++cc) {
+                const int o = cc % cnt;
+                a.col_sub[cc] = (int8_t)(cc / cnt);
+                a.col_op[cc] = (int8_t)o;
+                a.col_kind[cc] = (int8_t)a.op_kind[o];
+                a.col_k[cc] = (int8_t)a.op_k[o];
+            }
+            if (grouped) {
+                a.grp_x = grp_x;
+                a.grp_start = grp_start;
+                a.grp_count = grp_count;
+                a.ylist = ylist;
+                a.ntasks_dev = ntd;
+            }
+            a.scratch = mem;
+            a.mem_ld = mem_ld;
+            a.mem_tasks = mem_tasks;
+            a.colidx = o0 == 0 ? colidx : nullptr;
+            a.status = status;
+            a.status_accumulate = o0 > 0;
+            a.max_iter = 8;
+            a.tol = 4e-16;
+            if (tuned && precision == RBFFD_PREC_FP64) {
+                a.group = choose_group(c, a, may_tune);
+            } else if (tuned) {
+                int g[rbffd::kMaxGroups];
+                rbffd::solve_groups(n, P, precision, g);
+                a.group = g[0];
+            } else {
+                a.group = rbffd::gen_lanes_used(n, P, a.gen_cols);
+            }
+            if (o0 == 0) c->last_group = a.group;
+            rbffd::launch_solve(a, precision, s);
+            RBFFD_HIP(hipGetLastError());
         }
-        // right-hand-side columns of the generic kernel: row-major over
-        // (output row of the task, operator)
-        a.gen_cols = ky * cnt;
-        for (int cc = 0; cc < a.gen_cols; ++cc) {
-            const int o = cc % cnt;
-            a.col_sub[cc] = (int8_t)(cc / cnt);
-            a.col_op[cc] = (int8_t)o;
-            a.col_kind[cc] = (int8_t)a.op_kind[o];
-            a.col_k[cc] = (int8_t)a.op_k[o];
+    };
+    // A shape with a specialised kernel solves grouped tasks in the generic
+    // one, which can be slower than one specialised solve per row although it
+    // factors less (at 4 rows per X node and n = 31 it is).  The first large
+    // call per (shape, operators) times both and keeps the faster; smaller
+    // calls group.  RBFFD_GROUP_ROWS=1 forces grouping.
+    bool group = ky > 1;
+    if (group && rbffd::solve_tuned(n, P, precision)) {
+        const char *env = std::getenv("RBFFD_GROUP_ROWS");
+        const bool forced = env && env[0] == '1';
+        auto key = std::make_tuple(n, P, cmax);
+        auto it = c->group_rows.find(key);
+        if (forced) {
+            group = true;
+        } else if (it != c->group_rows.end()) {
+            group = it->second;
+        } else if (may_tune && M >= kGroupTuneRows && s == c->stream) {
+            for (auto &e : c->tune_ev)
+                if (!e) RBFFD_HIP(hipEventCreate(&e));
+            float ms[2] = {0, 0};
+            for (int v = 0; v < 2; ++v) {
+                RBFFD_HIP(hipEventRecord(c->tune_ev[0], s));
+                launch_all(v == 0 ? ky : 1);
+                RBFFD_HIP(hipEventRecord(c->tune_ev[1], s));
+                RBFFD_HIP(hipEventSynchronize(c->tune_ev[1]));
+                RBFFD_HIP(hipEventElapsedTime(&ms[v], c->tune_ev[0], c->tune_ev[1]));
+            }
+            group = ms[0] < ms[1];
+            c->group_rows[key] = group;
+            if (!group) {  // the last run was the faster one
+                if (ntasks_dev) *ntasks_dev = nullptr;
+                return ntask;
+            }
         }
-        a.grp_x = grp_x;
-        a.grp_start = grp_start;
-        a.grp_count = grp_count;
-        a.ylist = ylist;
-        a.ntasks_dev = ntd;
-        a.scratch = mem;
-        a.mem_ld = mem_ld;
-        a.mem_tasks = mem_tasks;
-        a.colidx = o0 == 0 ? colidx : nullptr;
-        a.status = status;
-        a.status_accumulate = o0 > 0;
-        a.max_iter = 8;
-        a.tol = 4e-16;
-        if (tuned && precision == RBFFD_PREC_FP64) {
-            a.group = choose_group(c, a, may_tune);
-        } else {
-            int g[rbffd::kMaxGroups];
-            rbffd::solve_groups(n, P, tuned ? precision : 0, g);
-            a.group = g[0];
-        }
-        if (o0 == 0) c->last_group = a.group;
-        rbffd::launch_solve(a, precision, s);
-        RBFFD_HIP(hipGetLastError());
     }
+    launch_all(group ? ky : 1);
+    if (!group && ntasks_dev) *ntasks_dev = nullptr;
     return ntask;
 }
 
@@ -1636,6 +1684,19 @@ int rbffd_ssprk3_closure_device(rbffd_context *c, int64_t N, const int64_t *rowp
                                    2.0 / 3.0, 2.0 / 3.0 * dt, s);
             close(u);
         }
+        RBFFD_HIP(hipGetLastError());
+    });
+}
+
+int rbffd_read_sum_device(rbffd_context *c, const double *x, int64_t n, double *out)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!x || !out || n < 0) throw Error{RBFFD_ERR_ARG, "read_sum: bad arguments"};
+        if (((uintptr_t)x & 15) != 0) throw Error{RBFFD_ERR_ARG, "read_sum: x must be 16-byte aligned"};
+        int cus = 0;
+        RBFFD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        rbffd::launch_read_sum(x, n, out, cus, c->stream);
         RBFFD_HIP(hipGetLastError());
     });
 }

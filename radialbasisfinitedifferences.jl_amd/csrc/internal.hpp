@@ -109,8 +109,10 @@ void launch_solve(const SolveArgs &a, int precision, hipStream_t s);
 constexpr int kMaxGroups = 3;
 void solve_groups(int n, int P, int precision, int *groups);
 // Output rows per task of the generic kernel for nops operators when an X node
-// has at most maxrows of them (1: no grouping).
-int gen_rows_per_task(int n, int P, int nops, int maxrows);
+// has at most maxrows of them and mean_rows on average (1: no grouping).
+int gen_rows_per_task(int n, int P, int nops, int maxrows, double mean_rows);
+// Lanes per stencil of the generic kernel for R right-hand-side columns.
+int gen_lanes_used(int n, int P, int R);
 // Workspace doubles per stencil of the generic kernel for systems above 128
 // equations (0 below): mem_ld rows by n + P + nops columns.
 long gen_mem_ld(int n, int P);
@@ -136,9 +138,17 @@ RBFFD_SHAPES(RBFFD_DECLARE_SHAPE)
 // Generic kernel per padded system size NSB (R: right-hand-side columns
 // needed): gen_lanes(NSB) lanes per stencil, right-hand-side blocks of 3, 6 and
 // gen_rb(NSB) columns (several output rows per task where that exceeds 6).
-#define RBFFD_GEN_BUCKETS(X) X(16) X(32) X(64) X(96) X(128)
+#define RBFFD_GEN_BUCKETS(X) X(16) X(32) X(48) X(64) X(96) X(128)
 constexpr int gen_lanes(int nsb) { return nsb <= 16 ? 16 : (nsb <= 32 ? 32 : 64); }
-constexpr int gen_rb(int nsb) { return nsb <= 32 ? 48 : (nsb <= 64 ? 72 : 6); }
+constexpr int gen_rb(int nsb) { return nsb <= 48 ? 48 : (nsb <= 64 ? 72 : 6); }
+// the narrow layout (two row slots per lane) for at most kGenNarrowRhs columns; 0: none
+constexpr int kGenNarrowRhs = 12;
+constexpr int gen_narrow_lanes(int nsb) { return nsb == 32 ? 16 : 0; }
+// lanes per stencil of the generic kernel for R right-hand-side columns
+constexpr int gen_lanes_for(int nsb, int R)
+{
+    return (gen_narrow_lanes(nsb) && R <= kGenNarrowRhs) ? gen_narrow_lanes(nsb) : gen_lanes(nsb);
+}
 #define RBFFD_DECLARE_GEN(nsb) void launch_gen_##nsb(const SolveArgs &a, int R, hipStream_t s);
 RBFFD_GEN_BUCKETS(RBFFD_DECLARE_GEN)
 #undef RBFFD_DECLARE_GEN
@@ -160,6 +170,8 @@ void launch_spmv(long M, const int64_t *rowptr, const int *col, const void *vals
                  const double *x, double *y, double alpha, double beta, int uniform_n, const int *order,
                  hipStream_t s);
 void launch_f64_to_f32(const double *src, long n, float *dst, hipStream_t s);
+// *out = sum of x[0 .. n) by a read-only stream over cus * 8 blocks (x 16-byte aligned)
+void launch_read_sum(const double *x, long n, double *out, int cus, hipStream_t s);
 
 // One SSP-RK stage out = a u + b w + c (L w) (rows with fixed != 0 keep u).
 void launch_rk_stage(long M, const int64_t *rowptr, const int *col, const void *vals, int vals_f32, int F,

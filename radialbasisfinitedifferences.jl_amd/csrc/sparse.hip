@@ -12,6 +12,7 @@
 // contiguous runs (coalesced), x is gathered per column, and the G partial
 // sums are reduced across lanes by DPP.  Fields are stored node-major
 // (x[col*F + f]) so one gather brings all F fields of a node.
+#include <algorithm>
 #include <cstdlib>
 
 #include <hipcub/hipcub.hpp>
@@ -548,6 +549,41 @@ void launch_closure(long nbc, int F, const double *G, const double *t, double *p
     default: throw Error{2, "closure: fields must be 1, 2, 4 or 8"};
     }
     closure_scatter_kernel<<<(unsigned)((nbc * F + 255) / 256), 256, 0, s>>>(nbc, F, nc, part, ghost, u);
+}
+
+// ------------------------------------------------------------ read ceiling
+// *out = sum of x[0 .. n): a read-only stream (16-byte loads, four in flight
+// per thread, one atomic per block) that measures the HBM read rate SpMV's
+// traffic (almost all reads) can reach.  *out is cleared first; the sum's last
+// bits depend on the order of the blocks' atomics.
+__global__ void clear_f64_kernel(double *out) { *out = 0.0; }
+
+__global__ __launch_bounds__(256) void read_sum_kernel(const double2 *x, long n2, const double *tail, int ntail,
+                                                        double *out)
+{
+    double acc = 0.0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n2; i += 4 * stride) {
+        const double2 a = x[i], b = x[i + stride], c = x[i + 2 * stride], d = x[i + 3 * stride];
+        acc += (a.x + a.y) + (b.x + b.y) + (c.x + c.y) + (d.x + d.y);
+    }
+    for (; i < n2; i += stride) acc += x[i].x + x[i].y;
+    if (blockIdx.x == 0 && (int)threadIdx.x < ntail) acc += tail[threadIdx.x];
+    __shared__ double part[4];
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, (part[0] + part[1]) + (part[2] + part[3]));
+}
+
+void launch_read_sum(const double *x, long n, double *out, int cus, hipStream_t s)
+{
+    clear_f64_kernel<<<1, 1, 0, s>>>(out);
+    if (n <= 0) return;
+    const long n2 = n / 2;
+    const unsigned blocks = (unsigned)std::max(1L, std::min<long>((long)cus * 8, (n2 + 255) / 256));
+    read_sum_kernel<<<blocks, 256, 0, s>>>((const double2 *)x, n2, x + 2 * n2, (int)(n - 2 * n2), out);
 }
 
 }  // namespace rbffd

@@ -16,7 +16,7 @@ constexpr int kGenMaxRows = 64;  // output rows per task (one status bit each)
 
 int gen_bucket(int nsys)
 {
-    const int sizes[] = {16, 32, 64, 96, 128};
+    const int sizes[] = {16, 32, 48, 64, 96, 128};
     for (int b : sizes)
         if (nsys <= b) return b;
     return 0;  // workspace-resident kernel
@@ -34,7 +34,7 @@ bool solve_tuned(int n, int P, int precision)
     return false;
 }
 
-int gen_rows_per_task(int n, int P, int nops, int maxrows)
+int gen_rows_per_task(int n, int P, int nops, int maxrows, double mean_rows)
 {
     const int b = gen_bucket(n + P);
     if (b == 0 || nops < 1 || maxrows <= 1) return 1;
@@ -43,9 +43,21 @@ int gen_rows_per_task(int n, int P, int nops, int maxrows)
     if (b == nsb) rb = gen_rb(nsb);
     RBFFD_GEN_BUCKETS(RBFFD_RB)
 #undef RBFFD_RB
-    int ky = rb / nops;
-    if (ky > maxrows) ky = maxrows;
+    // every X node in one task where that fits the narrow layout; else the
+    // narrow layout if an X node has about that many rows on average (the
+    // rest split into a few tasks); else the widest block
+    const int narrow = gen_narrow_lanes(b) ? kGenNarrowRhs / nops : 0;
+    int ky;
+    if (narrow >= maxrows) ky = maxrows;
+    else if (narrow >= 2 && narrow >= (int)(mean_rows + 0.999)) ky = narrow;
+    else ky = rb / nops < maxrows ? rb / nops : maxrows;
     return ky < 1 ? 1 : (ky > kGenMaxRows ? kGenMaxRows : ky);
+}
+
+int gen_lanes_used(int n, int P, int R)
+{
+    const int b = gen_bucket(n + P);
+    return b ? gen_lanes_for(b, R) : 64;
 }
 
 long gen_mem_ld(int n, int P)
@@ -89,7 +101,7 @@ void solve_groups(int n, int P, int precision, int *groups)
 #undef RBFFD_CASE
     }
     const int b = gen_bucket(n + P);
-    groups[0] = b ? gen_lanes(b) : 64;
+    groups[0] = b ? gen_lanes_for(b, kMaxRhs) : 64;
 }
 
 // ------------------------------------------------------------ grouping

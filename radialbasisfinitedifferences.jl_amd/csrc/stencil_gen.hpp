@@ -702,24 +702,31 @@ void launch_gen_t(const SolveArgs &a, hipStream_t s)
 }
 
 // Right-hand-side widths compiled per padded size: 3 and 6 (one output row per
-// task), then 12, 24, 48, 72 up to RB (several rows); the launch takes the
+// task), then 12, 24, 36, 48, 72 up to RB (several rows); the launch takes the
 // narrowest that holds R.
-template <int NSB, int G, int RB, int RC>
+// Up to kGenNarrowRhs columns, sizes with a narrow layout (GN lanes, two row
+// slots per lane) take it: twice the stencils per wavefront share each step's
+// pivot search and LDS round trip.
+template <int NSB, int G, int GN, int RB, int RC>
 void launch_gen_width(const SolveArgs &a, int R, hipStream_t s)
 {
     if constexpr (RC <= RB) {
-        if (R <= RC) { launch_gen_t<NSB, RC, G>(a, s); return; }
-        constexpr int NEXT = RC == 3 ? 6 : (RC == 6 ? 12 : (RC == 48 ? 72 : 2 * RC));
-        launch_gen_width<NSB, G, RB, NEXT>(a, R, s);
+        if (R <= RC) {
+            if constexpr (GN > 0 && RC <= kGenNarrowRhs) launch_gen_t<NSB, RC, GN>(a, s);
+            else launch_gen_t<NSB, RC, G>(a, s);
+            return;
+        }
+        constexpr int NEXT = RC == 3 ? 6 : (RC == 6 ? 12 : (RC == 12 ? 24 : (RC == 24 ? 36 : (RC == 36 ? 48 : 72))));
+        launch_gen_width<NSB, G, GN, RB, NEXT>(a, R, s);
     } else {
         throw Error{2, "too many right-hand sides for the generic kernel"};
     }
 }
 
-template <int NSB, int G, int RB>
+template <int NSB, int G, int GN, int RB>
 void launch_gen_bucket(const SolveArgs &a, int R, hipStream_t s)
 {
-    launch_gen_width<NSB, G, RB, 3>(a, R, s);
+    launch_gen_width<NSB, G, GN, RB, 3>(a, R, s);
 }
 
 }  // namespace rbffd
@@ -729,6 +736,6 @@ void launch_gen_bucket(const SolveArgs &a, int R, hipStream_t s)
     namespace rbffd {                                                                  \
     void launch_gen_##nsb(const SolveArgs &a, int R, hipStream_t s)                    \
     {                                                                                  \
-        launch_gen_bucket<nsb, gen_lanes(nsb), gen_rb(nsb)>(a, R, s);                  \
+        launch_gen_bucket<nsb, gen_lanes(nsb), gen_narrow_lanes(nsb), gen_rb(nsb)>(a, R, s);  \
     }                                                                                  \
     }

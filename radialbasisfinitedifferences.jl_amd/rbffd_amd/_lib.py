@@ -125,6 +125,7 @@ SIGNATURES = {
                                                ctypes.POINTER(_I64)]),
     "rbffd_spmv_device": (ctypes.c_int, [_V, _I64, _V, _V, _V, _I32, _I32, _V, _V, _D, _D, _I32, _V]),
     "rbffd_locality_order_device": (ctypes.c_int, [_V, _V, _I64, _V]),
+    "rbffd_read_sum_device": (ctypes.c_int, [_V, _V, _I64, _V]),
     "rbffd_lsq_solve_device": (ctypes.c_int, [_V, _I64, _I64, _V, _V, _V, _V, _V, _I32, _D, _I64,
                                               ctypes.POINTER(_I64), ctypes.POINTER(_D)]),
     "rbffd_combine_device": (ctypes.c_int, [_V, _I64, _I32, _V, _V, _V, _V]),

@@ -359,6 +359,13 @@ class Engine:
                                             float(tol), int(max_iters), ctypes.byref(it), ctypes.byref(rel)))
         return int(it.value), float(rel.value)
 
+    def read_sum(self, x, out=None):
+        """Device sum of x (float64) by a read-only stream; returns the 1-element
+        device array (asynchronous)."""
+        out = out if out is not None else self.ctx.empty((1,), np.float64)
+        check(self.L.rbffd_read_sum_device(self.ctx.handle, x.ptr, x.size, out.ptr))
+        return out
+
     def locality_order(self, X):
         """Morton order of the nodes of X (device, N x 2), as an int32 device array."""
         order = self.ctx.empty((X.shape[0],), np.int32)

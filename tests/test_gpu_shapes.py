@@ -4,7 +4,7 @@ stencil (Y distinct from X), on the GPU against the oracle.
 The reference accepts any stencil size n and polynomial degree
 (src/generate_operator.jl:29, src/polynomialbasis.jl:6).  The library solves
 them with a generic kernel that pads the system to a compiled size (16, 32,
-64, 96 or 128 equations) and, above 128 equations, with a kernel that keeps
+48, 64, 96 or 128 equations) and, above 128 equations, with a kernel that keeps
 the system in a device workspace.  With Y distinct from X the rows mapped to
 one X node are solved against one factorisation of its system
 (src/generate_operator.jl:55-65, :158).
@@ -34,7 +34,7 @@ OPS = ["E", "Dx", "Dy", "Dxx", "Dyy", "Dxy", "Dxk", "Dyk", "Lap", "Lapk"]
 SHAPES = [
     (3, 1, 7, 16),      # n + P = 10
     (8, 1, 21, 32),     # even power, 24
-    (5, 3, 25, 64),     # 35
+    (5, 3, 25, 48),     # 35
     (7, 4, 35, 64),     # 50
     (3, 4, 40, 64),     # 55
     (5, 5, 50, 96),     # 71
