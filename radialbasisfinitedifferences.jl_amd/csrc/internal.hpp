@@ -153,6 +153,11 @@ constexpr int gen_lanes_for(int nsb, int R)
 RBFFD_GEN_BUCKETS(RBFFD_DECLARE_GEN)
 #undef RBFFD_DECLARE_GEN
 void launch_gen_mem(const SolveArgs &a, hipStream_t s);
+// Two wavefronts per stencil (one row per lane) for the 96- and 128-equation
+// sizes, at most 24 right-hand-side columns.
+void launch_gen2_96(const SolveArgs &a, int R, hipStream_t s);
+void launch_gen2_128(const SolveArgs &a, int R, hipStream_t s);
+constexpr int kGen2MaxRhs = 24;
 
 // --------------------------------------------------- sparse (K3, K4)
 void launch_fill_rowptr(int64_t *rowptr, long M, int n, hipStream_t s);

@@ -3,11 +3,21 @@
 // generic kernel (stencil_gen<NSB>.hip, stencil_genmem.hip) for every other
 // shape and for output rows that share a stencil.  Also the grouping of Y
 // rows by their stencil.
+#include <cstdlib>
 #include <string>
 
 #include "internal.hpp"
 
 namespace rbffd {
+
+// RBFFD_SPECIALISED=0 sends every shape to the generic kernels and
+// RBFFD_GEN_TWO_WAVES=0 keeps the 96- and 128-equation sizes in one
+// wavefront per stencil (both for comparisons).
+static bool env_off(const char *name)
+{
+    const char *e = std::getenv(name);
+    return e && e[0] == '0';
+}
 
 namespace {
 constexpr int kGenMaxN = 128;    // stencil nodes: the neighbour search's limit
@@ -27,6 +37,7 @@ bool solve_supported(int n, int P) { return n >= 1 && n <= kGenMaxN && P >= 1 &&
 
 bool solve_tuned(int n, int P, int precision)
 {
+    if (env_off("RBFFD_SPECIALISED")) return false;
 #define RBFFD_CASE(nn, pp) \
     if (n == nn && P == pp) return precision == 0 || nn + pp <= 64;
     RBFFD_SHAPES(RBFFD_CASE)
@@ -43,6 +54,7 @@ int gen_rows_per_task(int n, int P, int nops, int maxrows, double mean_rows)
     if (b == nsb) rb = gen_rb(nsb);
     RBFFD_GEN_BUCKETS(RBFFD_RB)
 #undef RBFFD_RB
+    if ((b == 96 || b == 128) && !env_off("RBFFD_GEN_TWO_WAVES")) rb = kGen2MaxRhs;
     // every X node in one task where that fits the narrow layout; else the
     // narrow layout if an X node has about that many rows on average (the
     // rest split into a few tasks); else the widest block
@@ -57,6 +69,7 @@ int gen_rows_per_task(int n, int P, int nops, int maxrows, double mean_rows)
 int gen_lanes_used(int n, int P, int R)
 {
     const int b = gen_bucket(n + P);
+    if ((b == 96 || b == 128) && R <= kGen2MaxRhs && !env_off("RBFFD_GEN_TWO_WAVES")) return 128;
     return b ? gen_lanes_for(b, R) : 64;
 }
 
@@ -83,6 +96,11 @@ void launch_solve(const SolveArgs &a, int precision, hipStream_t s)
         if (a.grp_x || !a.scratch || a.mem_ld < a.n + a.P || a.gen_cols > kMaxRhs)
             throw Error{3, "workspace-resident solve: bad launch arguments"};
         launch_gen_mem(a, s);
+        return;
+    }
+    if ((b == 96 || b == 128) && a.gen_cols <= kGen2MaxRhs && !env_off("RBFFD_GEN_TWO_WAVES")) {
+        if (b == 96) launch_gen2_96(a, a.gen_cols, s);
+        else launch_gen2_128(a, a.gen_cols, s);
         return;
     }
 #define RBFFD_GEN(nsb) \
