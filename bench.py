@@ -93,7 +93,13 @@ def main():
     ap.add_argument("--extras", default="all",
                     help="comma-separated subset of the extra workloads: hyperviscosity_4m, uneven_batch_64, "
                          "advdiff_ssprk3_4m, poisson_solve, y_distinct_4m (default all)")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="profiling workload: generate the Laplacian of the 1M-node set, then 10 products "
+                         "with 1 field and 10 with 8 fields; prints their mean times")
     args = ap.parse_args()
+    if args.spmv_only:
+        spmv_only()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -316,6 +322,28 @@ def main():
         "end_to_end": e2e, "accuracy": accuracy,
         "spmv": spmv, "mixed": mixed, "cpu_baseline": cpu, "extras": extras, "device": info,
     }
+    print(json.dumps(out))
+
+
+def spmv_only():
+    import rbffd_amd
+    from rbffd_amd.nodes import config_nodes
+
+    ctx = rbffd_amd.Context(0)
+    eng = rbffd_amd.Engine(ctx)
+    X, prm = config_nodes("gen1m")
+    N, n = len(X), prm["n"]
+    Xd = ctx.to_device(X)
+    col, vals, _ = eng.generate(Xd, Xd, prm["p"], n, prm["polydeg"], ["Lap"])
+    out = {}
+    for F in (1, 8):
+        x = ctx.to_device(np.random.default_rng(F).standard_normal((N, F)))
+        y = ctx.empty((N, F), np.float64)
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(10):
+            eng.spmv(col, vals[0], x, y, uniform_n=n)
+        out[f"F{F}_ms"] = ctx.timer_stop() / 10
     print(json.dumps(out))
 
 
