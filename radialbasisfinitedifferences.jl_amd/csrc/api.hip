@@ -31,7 +31,7 @@ struct rbffd_buf {
 // Workspace slots of one neighbour search and the per-set buffers of a batch.
 enum ArenaSlot {
     A_PART, A_BOX, A_GRID, A_COUNTS, A_START, A_CELL_OF, A_FILL, A_CELL_XY, A_CELL_IDX, A_CELL_GRP,
-    A_SCAN_TMP, A_STENCILS, A_STATUS, A_REDO, A_GEN_SCRATCH, A_NSLOT
+    A_SCAN_TMP, A_STENCILS, A_STATUS, A_REDO, A_REDO_TAU, A_REDO2, A_GEN_SCRATCH, A_NSLOT
 };
 
 struct rbffd_arena {
@@ -363,7 +363,12 @@ void knn_enqueue(rbffd_arena &A, bool may_alloc, const double *X, long N, const 
     // X queries run in cell order so a wavefront walks neighbouring cells
     const int *qorder = (Q == X && M == N) ? cidx : nullptr;
     int *redo = (int *)grow(A.b[A_REDO], sizeof(int) * (M + 1), may_alloc);
-    rbffd::launch_knn_query(grid, start, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, redo + 1, redo, s);
+    rbffd::KnnRedo rw;
+    rw.tau = (double *)grow(A.b[A_REDO_TAU], sizeof(double) * M, may_alloc);
+    int *redo2 = (int *)grow(A.b[A_REDO2], sizeof(int) * (M + 1), may_alloc);
+    rw.count2 = redo2;
+    rw.list2 = redo2 + 1;
+    rbffd::launch_knn_query(grid, start, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, redo + 1, redo, rw, s);
     count_negative_kernel<<<nblk(M * (long)k), 256, 0, s>>>(idx, M * (long)k, flag + 1);
     RBFFD_HIP(hipGetLastError());
 }

@@ -54,10 +54,18 @@ void launch_cell_scatter(const double *X, const int *xgroup, long N, const int *
                          int *fill, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s);
 // Exact kNN of every query, ascending (d^2, index).  redo (M ints, may be
 // NULL) and redo_count (one int) are the workspace of the two-phase selection
-// used for k > 8; without them every query runs the insertion kernel.
+// used for k > 8; without them every query runs the insertion kernel.  With
+// them, rw holds the queries' k-th distances (M doubles) and the list (M ints)
+// and count (one int) of queries left to the insertion kernel.
+struct KnnRedo {
+    double *tau;
+    int *list2;
+    int *count2;
+};
 void launch_knn_query(const GridParams *g, const int *cell_start, const double *cell_xy, const int *cell_idx,
                       const int *cell_grp, const double *Q, const int *qgroup, const int *qorder, long M,
-                      int k, int *idx_out, double *d2_out, int *redo, int *redo_count, hipStream_t s);
+                      int k, int *idx_out, double *d2_out, int *redo, int *redo_count, const KnnRedo &rw,
+                      hipStream_t s);
 
 // ------------------------------------------------------- local solves (K2)
 struct SolveArgs {

@@ -10,6 +10,8 @@
 // distance dx*dx + dy*dy (never fused), ties by lower index, so the result
 // does not depend on the order in which cells or points are visited.
 #include <hipcub/hipcub.hpp>
+
+#include <algorithm>
 #include <limits.h>
 
 #include "device_common.hpp"
@@ -326,7 +328,8 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
                                                          const int *__restrict__ qgroup,
                                                          const int *__restrict__ qorder, long M, int k,
                                                          int *__restrict__ idx_out, double *__restrict__ d2_out,
-                                                         int *__restrict__ redo, int *__restrict__ redo_count)
+                                                         int *__restrict__ redo, double *__restrict__ redo_tau,
+                                                         int *__restrict__ redo_count)
 {
 #pragma clang fp contract(off)
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -334,7 +337,9 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
     const GridParams g = *gp;
     const long q = qorder ? (long)qorder[t] : t;
     if (!g.valid) {
-        redo[atomicAdd(redo_count, 1)] = (int)q;
+        const int slot = atomicAdd(redo_count, 1);
+        redo[slot] = (int)q;
+        redo_tau[slot] = INFINITY;
         return;
     }
     const double qx = Q[2 * q], qy = Q[2 * q + 1];
@@ -438,19 +443,135 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
             });
         }
     }
-    if (tie || nsel != k) redo[atomicAdd(redo_count, 1)] = (int)q;
+    if (tie || nsel != k) {
+        const int slot = atomicAdd(redo_count, 1);
+        redo[slot] = (int)q;
+        redo_tau[slot] = tie ? tau : (nsel > k ? tau : INFINITY);
+    }
+}
+
+// Queries the two-phase selection could not place (an exact distance tie, or
+// more than k candidates at d <= tau): one wavefront per query.  tau, the
+// exact k-th smallest distance from phase 1, bounds the cell walk as it did
+// there; the lanes split each ring's cells and append every visible candidate
+// with d <= tau to an LDS list, and each candidate's rank in the (d^2, index)
+// order is counted against the list.  The first k ranks are the answer.
+// Queries without a finite tau (fewer than k visible nodes, non-finite
+// points) or with more than kTieCap candidates go on to the insertion kernel.
+constexpr int kTieCap = 256;
+constexpr int kTieWaves = 4;
+
+__global__ __launch_bounds__(64 * kTieWaves) void knn_tie_kernel(
+    const GridParams *__restrict__ gp, const int *__restrict__ cell_start, const double *__restrict__ cell_xy,
+    const int *__restrict__ cell_idx, const int *__restrict__ cell_grp, const double *__restrict__ Q,
+    const int *__restrict__ qgroup, const int *__restrict__ redo, const double *__restrict__ redo_tau,
+    const int *__restrict__ redo_count, int k, int *__restrict__ idx_out, double *__restrict__ d2_out,
+    int *__restrict__ redo2, int *__restrict__ redo2_count)
+{
+#pragma clang fp contract(off)
+    __shared__ double cd[kTieWaves][kTieCap];
+    __shared__ int ci[kTieWaves][kTieCap];
+    __shared__ int cnt[kTieWaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const GridParams g = *gp;
+    const long nred = *redo_count;
+    const long stride = (long)gridDim.x * kTieWaves;
+    for (long e = (long)blockIdx.x * kTieWaves + w; e < nred; e += stride) {  // uniform per wavefront
+        const long q = redo[e];
+        const double tau = redo_tau[e];
+        if (!g.valid || !(tau < INFINITY)) {
+            if (lane == 0) redo2[atomicAdd(redo2_count, 1)] = (int)q;
+            continue;
+        }
+        const double qx = Q[2 * q], qy = Q[2 * q + 1];
+        const int qg = (cell_grp && qgroup) ? qgroup[q] : -1;
+        const double fx = (qx - g.x0) / g.csx, fy = (qy - g.y0) / g.csy;
+        const int qcx = (int)fmin(fmax(floor(fx), -2.0), (double)g.gx + 1.0);
+        const int qcy = (int)fmin(fmax(floor(fy), -2.0), (double)g.gy + 1.0);
+        const int rlimit = g.gx + g.gy + 4;
+        if (lane == 0) cnt[w] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (int r = 0; r <= rlimit; ++r) {
+            const int ncell = r == 0 ? 1 : 8 * r;
+            for (int tc = lane; tc < ncell; tc += 64) {
+                int dxc = 0, dyc = 0;
+                if (r > 0) {
+                    const int side = tc / (2 * r), off = tc - side * 2 * r;
+                    if (side == 0) { dxc = -r + off; dyc = -r; }
+                    else if (side == 1) { dxc = r; dyc = -r + off; }
+                    else if (side == 2) { dxc = r - off; dyc = r; }
+                    else { dxc = -r; dyc = r - off; }
+                }
+                const int cx = qcx + dxc, cy = qcy + dyc;
+                if (cx < 0 || cx >= g.gx || cy < 0 || cy >= g.gy) continue;
+                const double rx0 = g.x0 + cx * g.csx, rx1 = rx0 + g.csx;
+                const double ry0 = g.y0 + cy * g.csy, ry1 = ry0 + g.csy;
+                const double ddx = qx < rx0 ? rx0 - qx : (qx > rx1 ? qx - rx1 : 0.0);
+                const double ddy = qy < ry0 ? ry0 - qy : (qy > ry1 ? qy - ry1 : 0.0);
+                if ((ddx * ddx + ddy * ddy) * (1.0 - 1e-9) > tau) continue;
+                const int c = cy * g.gx + cx;
+                const int p1 = cell_start[c + 1];
+                for (int p = cell_start[c]; p < p1; ++p) {
+                    if (qg >= 0) {
+                        const int h = cell_grp[p];
+                        if (h != 0 && h != qg) continue;
+                    }
+                    const double dx = cell_xy[2 * (long)p] - qx, dy = cell_xy[2 * (long)p + 1] - qy;
+                    const double dd = dx * dx;
+                    const double ee = dy * dy;
+                    const double d = dd + ee;
+                    if (d <= tau) {
+                        const int slot = atomicAdd(&cnt[w], 1);
+                        if (slot < kTieCap) {
+                            cd[w][slot] = d;
+                            ci[w][slot] = cell_idx[p];
+                        }
+                    }
+                }
+            }
+            const double bx0 = qx - (g.x0 + (double)(qcx - r) * g.csx);
+            const double bx1 = (g.x0 + (double)(qcx + r + 1) * g.csx) - qx;
+            const double by0 = qy - (g.y0 + (double)(qcy - r) * g.csy);
+            const double by1 = (g.y0 + (double)(qcy + r + 1) * g.csy) - qy;
+            const double b = fmin(fmin(bx0, bx1), fmin(by0, by1));
+            if (b > 0.0 && b * b * (1.0 - 1e-9) > tau) break;
+            if (qcx - r <= 0 && qcy - r <= 0 && qcx + r >= g.gx - 1 && qcy + r >= g.gy - 1) break;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int C = cnt[w];
+        if (C > kTieCap || C < k) {
+            if (lane == 0) redo2[atomicAdd(redo2_count, 1)] = (int)q;
+            continue;
+        }
+        for (int i = lane; i < C; i += 64) {
+            const double di = cd[w][i];
+            const int ii = ci[w][i];
+            int rank = 0;
+            for (int j = 0; j < C; ++j) rank += key_less(cd[w][j], ci[w][j], di, ii) ? 1 : 0;
+            if (rank < k) {
+                idx_out[q * k + rank] = ii;
+                if (d2_out) d2_out[q * k + rank] = di;
+            }
+        }
+        // the next query's count starts at 0 only after every lane has read this one
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 template <int KMAX>
 static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
                          const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
-                         double *d2, int *redo, int *redo_count, hipStream_t s)
+                         double *d2, int *redo, int *redo_count, const KnnRedo &rw, hipStream_t s)
 {
     const unsigned blocks = (unsigned)((M + 255) / 256);
     if (KMAX >= 16 && redo) {
         knn_select_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2,
-                                                        redo, redo_count);
-        knn_query_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, redo, M, redo_count, k,
+                                                        redo, rw.tau, redo_count);
+        // at most 16384 wavefronts walk the tied queries in turn
+        const unsigned tblocks = (unsigned)std::max(1L, std::min<long>((M + kTieWaves - 1) / kTieWaves, 4096));
+        knn_tie_kernel<<<tblocks, 64 * kTieWaves, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, redo, rw.tau,
+                                                           redo_count, k, idx, d2, rw.list2, rw.count2);
+        knn_query_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, rw.list2, M, rw.count2, k,
                                                        idx, d2);
     } else {
         knn_query_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, nullptr, k,
@@ -460,11 +581,15 @@ static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, 
 
 void launch_knn_query(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
                       const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
-                      double *d2, int *redo, int *redo_count, hipStream_t s)
+                      double *d2, int *redo, int *redo_count, const KnnRedo &rw, hipStream_t s)
 {
     if (M <= 0) return;
-    if (redo) launch_fill_i32(redo_count, 1, 0, s);
-#define RBFFD_KNN(KM) knn_launch_t<KM>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, redo, redo_count, s)
+    if (redo) {
+        launch_fill_i32(redo_count, 1, 0, s);
+        launch_fill_i32(rw.count2, 1, 0, s);
+    }
+#define RBFFD_KNN(KM) \
+    knn_launch_t<KM>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, redo, redo_count, rw, s)
     if (k <= 1) RBFFD_KNN(1);
     else if (k <= 8) RBFFD_KNN(8);
     else if (k <= 16) RBFFD_KNN(16);
