@@ -379,6 +379,18 @@ constexpr int lu_wpb(int bytes)
     return best;
 }
 
+// Wavefronts per SIMD the FP64 Gauss-Jordan kernel is compiled for: three
+// (168 registers) where the row registers, ROWS x WIDTH doubles, leave the
+// headroom the rest of the kernel needs at G lanes per stencil (measured on
+// the compiled shapes: 34 registers at 64 lanes, 44 at 16), else no bound.
+template <int NN, int NP, int R, int G>
+constexpr int lu_min_waves()
+{
+    using T = LuShape<NN, NP, R, G>;
+    constexpr int head = G == 16 ? 44 : (G == 32 ? 40 : 34);
+    return T::ROWS * T::WIDTH * 2 + head <= 168 ? 3 : 1;
+}
+
 // FP64 Gauss-Jordan kernel: G lanes per stencil.
 template <int NN, int NP, int R, int G>
 void launch_lu(const SolveArgs &a, hipStream_t s)
@@ -390,16 +402,22 @@ void launch_lu(const SolveArgs &a, hipStream_t s)
     constexpr int wpb = G == 16 ? 1 : lu_wpb(bytes);
     constexpr int per_block = wpb * (64 / G);
     const unsigned blocks = (unsigned)((a.ntasks + per_block - 1) / per_block);
-    solve_lu_kernel<NN, NP, R, G, wpb, 1><<<blocks, 64 * wpb, 0, s>>>(a);
+    solve_lu_kernel<NN, NP, R, G, wpb, lu_min_waves<NN, NP, R, G>()><<<blocks, 64 * wpb, 0, s>>>(a);
 }
 
-// Lane-group sizes compiled for a system size: g0 is the default, the others
-// (0 = none) are candidates for the autotuner.
-template <int NS>
+// Lane-group sizes compiled for a stencil size: g0 is the default, the others
+// (0 = none) are candidates for the autotuner.  A lane holds ceil(n / G) row
+// slots of n + P + r doubles; a layout is compiled where those stay within
+// 200 registers at r = 3 (no spill with the rest of the kernel) and where at
+// least half of a group's lanes hold a node.  g0 is the smallest such G
+// (most stencils per wavefront), 64 when none fits.
+template <int NN, int NS>
 struct LuGroups {
-    static constexpr int g0 = NS <= 32 ? 16 : (NS <= 64 ? 32 : 64);
-    static constexpr int g1 = NS <= 48 ? (NS <= 32 ? 32 : 16) : (NS <= 64 ? 64 : 0);
-    static constexpr int g2 = (NS > 32 && NS <= 48) ? 64 : 0;
+    static constexpr bool fits(int g) { return ((NN + g - 1) / g) * (NS + 3) * 2 <= 200; }
+    static constexpr bool used(int g) { return fits(g) && 2 * NN > g; }
+    static constexpr int g0 = used(16) ? 16 : (used(32) ? 32 : 64);
+    static constexpr int g1 = g0 < 32 && used(32) ? 32 : (g0 < 64 && used(64) ? 64 : 0);
+    static constexpr int g2 = g0 < 32 && g1 == 32 && used(64) ? 64 : 0;
 };
 
 template <int NN, int NP, int R>
@@ -410,7 +428,7 @@ void launch_prec(const SolveArgs &a, int precision, hipStream_t s)
         if constexpr (NS <= 64) launch_t<float, NN, NP, R, true>(a, s);
         else throw Error{2, "mixed precision needs n + P <= 64"};
     } else {
-        constexpr int g0 = LuGroups<NS>::g0, g1 = LuGroups<NS>::g1, g2 = LuGroups<NS>::g2;
+        constexpr int g0 = LuGroups<NN, NS>::g0, g1 = LuGroups<NN, NS>::g1, g2 = LuGroups<NN, NS>::g2;
         if constexpr (g1 != 0) {
             if (a.group == g1) { launch_lu<NN, NP, R, g1>(a, s); return; }
         }
@@ -436,9 +454,9 @@ template <int NN, int NP>
 void shape_groups(int precision, int *groups)
 {
     constexpr int NS = NN + NP;
-    groups[0] = precision == 1 ? 64 : LuGroups<NS>::g0;
-    groups[1] = precision == 1 ? 0 : LuGroups<NS>::g1;
-    groups[2] = precision == 1 ? 0 : LuGroups<NS>::g2;
+    groups[0] = precision == 1 ? 64 : LuGroups<NN, NS>::g0;
+    groups[1] = precision == 1 ? 0 : LuGroups<NN, NS>::g1;
+    groups[2] = precision == 1 ? 0 : LuGroups<NN, NS>::g2;
 }
 
 }  // namespace rbffd

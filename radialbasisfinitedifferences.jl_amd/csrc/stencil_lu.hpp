@@ -251,37 +251,39 @@ __device__ __forceinline__ double op_phs_fast(int op, int k, int m, double vx, d
 template <int NN, int NP, int R, int G>
 struct LuShape {
     static constexpr int NS = NN + NP;
-    static constexpr int ROWS = (NS + G - 1) / G;  // rows per lane
+    // Register row slots per lane: the RBF equations (slot rr of lane t holds
+    // equation t + G rr, this lane's node rr).  The lambda columns are
+    // eliminated first, highest degree first, and only the RBF equations are
+    // nonzero there: the first NP steps run on them alone.  A lambda pivot row
+    // is not needed after those steps (it only gives lambda), so the monomial
+    // equations then take the slots of the NP lambda pivots (and of empty
+    // slots), and the remaining NN steps run on the same ROWS slots.
+    static constexpr int ROWS = (NN + G - 1) / G;
     static constexpr int WIDTH = NS + R;
     static constexpr int S = 64 / G;               // stencils per wavefront
-    static constexpr int NODES = (NN + G - 1) / G; // stencil nodes per lane
+    static constexpr int NODES = ROWS;             // stencil nodes per lane
+    static constexpr int QR = (NP + G - 1) / G;    // monomial equations per lane during assembly
     static constexpr int SB = step_block<G>();
-    // Monomial columns first: when the RBF equations fill whole row slots of
-    // their own (slots 0..R1-1) and the monomial equations the rest, the
-    // lambda columns are eliminated first, highest degree first.  The
-    // monomial equations are zero in those columns, so for the first NP steps
-    // their slots are neither updated nor searched, and they are loaded only
-    // when step NP-1 begins.
-    static constexpr int R1 = (NN + G - 1) / G;
-    static constexpr bool PFIRST = R1 + (NP + G - 1) / G == ROWS;
     // the current pivot row: one buffer, reused by every step (LDS serves a
     // wavefront's operations in order, so a step's reads finish before the
     // next step's writes).  A step writes up to SB-1 stale values past its
     // row and reads the row in whole runs of 16.
     static constexpr int UBUF = 16 * ((WIDTH + 15) / 16);
     // during assembly: Phi as a packed upper triangle and the scaled node
-    // coordinates (x, y) pairs.  P (node-major) follows them when the P^T
-    // rows are built with the others; with PFIRST it is written once the rows
-    // are built, over the dead triangle past the pivot-row buffer, and read
-    // when the monomial equations are loaded.
+    // coordinates (x, y) pairs.  Once the rows are built, P (node-major) and
+    // the monomial equations' right-hand sides (NP x R) go over the dead
+    // triangle past the pivot-row buffer, to be read when the monomial
+    // equations are loaded.
     static constexpr int PHI0 = 0;
     static constexpr int XI0 = PHI0 + NN * (NN + 1) / 2;
-    static constexpr int PB0 = PFIRST ? UBUF : XI0 + 2 * NN;
-    static constexpr int ASM = cmax_<XI0 + 2 * NN, PB0 + NN * NP>::v;
+    static constexpr int PB0 = UBUF;
+    static constexpr int MR0 = PB0 + NN * NP;
+    static constexpr int ASM = cmax_<XI0 + 2 * NN, MR0 + NP * R>::v;
     // Phi(a, b), a <= b, at PHI0 + tri(a) + b
     static constexpr __host__ __device__ int tri(int a) { return a * (2 * NN - a + 1) / 2 - a; }
-    // +2 keeps the groups' slots off one LDS bank
-    static constexpr int USTRIDE = cmax_<UBUF, ASM>::v + 2;
+    // even, so every group's row buffer is 16-byte aligned; +2 keeps the
+    // groups' slots off one LDS bank
+    static constexpr int USTRIDE = ((cmax_<UBUF, ASM>::v + 1) & ~1) + 2;
 };
 
 template <int NN, int NP, int R, int G>
@@ -299,7 +301,6 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     using T = LuShape<NN, NP, R, G>;
     constexpr int NS = T::NS, ROWS = T::ROWS, WIDTH = T::WIDTH, S = T::S, NODES = T::NODES;
     static_assert(NS <= 128, "system size above 128");
-    static_assert(!T::PFIRST || T::R1 == NODES, "RBF row slot rr holds this lane's node rr");
     __shared__ LuLDS<NN, NP, R, G> lds_all[WPB];
 
     const int lane = threadIdx.x & 63;
@@ -385,22 +386,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         for (int r = 0; r < NODES; ++r)
             if (t + G * r < NN) L.rank[grp][t + G * r] = rk[r];
     }
-    // monomial block P (node-major) for the transposed rows; with PFIRST the
-    // rows are built first and P follows from their registers
     constexpr int DEG = mono_deg(NP - 1);
-#pragma unroll
-    for (int r = 0; r < (T::PFIRST ? 0 : NODES); ++r) {
-        const int i = t + G * r;
-        if (i < NN) {
-            double pxp[DEG + 1], pyp[DEG + 1];
-            pxp[0] = 1.0;
-            pyp[0] = 1.0;
-#pragma unroll
-            for (int d = 1; d <= DEG; ++d) { pxp[d] = pxp[d - 1] * dxn[r]; pyp[d] = pyp[d - 1] * dyn[r]; }
-#pragma unroll
-            for (int q = 0; q < NP; ++q) L.u[grp][T::PB0 + i * NP + q] = pxp[mono_ea(q)] * pyp[mono_eb(q)];
-        }
-    }
     // Phi_ab = |xi_a - xi_b|^m once per pair: node i takes the pairs (i, i+d mod n)
     // for d = 1..n/2, which covers each pair once (twice for d = n/2, n even)
     {
@@ -437,26 +423,19 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     }
     __builtin_amdgcn_wave_barrier();
 
-    // ---- equations by row slot.  Lane t's slot rr holds RBF equation i
-    // (src/rbfblock.jl:14-20 with the P block) or monomial equation q (the
-    // P^T rows); eq_of gives i, NN + q, or NS for an empty slot.
+    // ---- equations by row slot.  Lane t's slot rr starts with RBF equation
+    // i = t + G rr (src/rbfblock.jl:14-20 with the P block); eq_of gives i, or
+    // NS for an empty slot.  The monomial equations (the P^T rows) enter the
+    // slots later, when the lambda columns are done.
     auto eq_of = [&](int rr) -> int {
-        if constexpr (T::PFIRST) {
-            if (rr < T::R1) {
-                const int i = t + G * rr;
-                return i < NN ? i : NS;
-            }
-            const int q = t + G * (rr - T::R1);
-            return q < NP ? NN + q : NS;
-        } else {
-            const int i = t + G * rr;
-            return i < NS ? i : NS;
-        }
+        const int i = t + G * rr;
+        return i < NN ? i : NS;
     };
 
     // ---- right-hand sides first, while few registers are live: RBF rows at
     // u = Yscaled - xi with exact zeros -> eps (src/generate_operator.jl:125-154),
-    // polynomial rows at Yscaled (:68-79)
+    // polynomial rows at Yscaled (:68-79); lane t computes monomial equation
+    // t + G r for its slots' later tenants
     double rhs[ROWS][R];
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
@@ -483,51 +462,45 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
 #pragma unroll
             for (int cc = 0; cc < R; ++cc)
                 if (cc < a.nops) rhs[rr][cc] = op_phs_fast(a.op_kind[cc], a.op_k[cc], m, ux, uy, sx, sy, q);
-        } else if (e < NS) {
-            const int q = e - NN;
+        }
+    }
+    double mrhs[T::QR][R];
+#pragma unroll
+    for (int r = 0; r < T::QR; ++r) {
+        const int q = t + G * r;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) mrhs[r][cc] = 0.0;
+        if (q < NP) {
             int tt = 0;
             while ((tt + 1) * (tt + 2) / 2 <= q) ++tt;
             const int ea = q - tt * (tt + 1) / 2, eb = tt - ea;
 #pragma unroll
             for (int cc = 0; cc < R; ++cc)
-                if (cc < a.nops) rhs[rr][cc] = op_mono(a.op_kind[cc], a.op_k[cc], ea, eb, yx, yy, sx, sy);
+                if (cc < a.nops) mrhs[r][cc] = op_mono(a.op_kind[cc], a.op_k[cc], ea, eb, yx, yy, sx, sy);
         }
     }
 
     // ---- rows of [A | B] (src/interpolationmatrix.jl:2-5, src/generate_operator.jl:157).
-    // Column order: w then lambda; with PFIRST lambda (highest degree first)
-    // then w.  Either way the right-hand sides follow at NS.
+    // Column order: lambda (highest degree first), then w, then the
+    // right-hand sides at NS.
     double row[ROWS][WIDTH];
     const double *phi = L.u[grp] + T::PHI0;
     const double *pblk = L.u[grp] + T::PB0;  // P(node j, monomial q) at j * NP + q
 #pragma unroll
-    for (int rr = 0; rr < (T::PFIRST ? T::R1 : ROWS); ++rr) {
+    for (int rr = 0; rr < ROWS; ++rr) {
         const int e = eq_of(rr);
-        constexpr int W0 = T::PFIRST ? NP : 0;  // first w column
-        constexpr int L0 = T::PFIRST ? 0 : NN;  // first lambda column
         if (e < NN) {
             const int rbase = T::tri(e);  // Phi(e, j) for j >= e
 #pragma unroll
-            for (int j = 0; j < NN; ++j) row[rr][W0 + j] = phi[j >= e ? rbase + j : T::tri(j) + e];
-            if constexpr (T::PFIRST) {
-                // node e = t + G rr is this lane's node rr: its monomials, highest first
-                double pxp[DEG + 1], pyp[DEG + 1];
-                pxp[0] = 1.0;
-                pyp[0] = 1.0;
+            for (int j = 0; j < NN; ++j) row[rr][NP + j] = phi[j >= e ? rbase + j : T::tri(j) + e];
+            // node e = t + G rr is this lane's node rr: its monomials, highest first
+            double pxp[DEG + 1], pyp[DEG + 1];
+            pxp[0] = 1.0;
+            pyp[0] = 1.0;
 #pragma unroll
-                for (int d = 1; d <= DEG; ++d) { pxp[d] = pxp[d - 1] * dxn[rr]; pyp[d] = pyp[d - 1] * dyn[rr]; }
+            for (int d = 1; d <= DEG; ++d) { pxp[d] = pxp[d - 1] * dxn[rr]; pyp[d] = pyp[d - 1] * dyn[rr]; }
 #pragma unroll
-                for (int q = 0; q < NP; ++q) row[rr][q] = pxp[mono_ea(NP - 1 - q)] * pyp[mono_eb(NP - 1 - q)];
-            } else {
-#pragma unroll
-                for (int q = 0; q < NP; ++q) row[rr][L0 + q] = pblk[e * NP + q];
-            }
-        } else if (e < NS) {
-            const int q = e - NN;
-#pragma unroll
-            for (int j = 0; j < NN; ++j) row[rr][W0 + j] = pblk[j * NP + q];
-#pragma unroll
-            for (int j = 0; j < NP; ++j) row[rr][L0 + j] = 0.0;
+            for (int q = 0; q < NP; ++q) row[rr][q] = pxp[mono_ea(NP - 1 - q)] * pyp[mono_eb(NP - 1 - q)];
         } else {
 #pragma unroll
             for (int j = 0; j < NS; ++j) row[rr][j] = 0.0;
@@ -535,36 +508,24 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) row[rr][NS + cc] = rhs[rr][cc];
     }
-    if constexpr (T::PFIRST) {
-        // P for the monomial equations, over the triangle just read
-        __builtin_amdgcn_wave_barrier();
+    // P and the monomial equations' right-hand sides, over the triangle just read
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int rr = 0; rr < T::R1; ++rr) {
-            const int e = eq_of(rr);
-            if (e < NN) {
+    for (int rr = 0; rr < ROWS; ++rr) {
+        const int e = eq_of(rr);
+        if (e < NN) {
 #pragma unroll
-                for (int q = 0; q < NP; ++q) L.u[grp][T::PB0 + e * NP + (NP - 1 - q)] = row[rr][q];
-            }
+            for (int q = 0; q < NP; ++q) L.u[grp][T::PB0 + e * NP + (NP - 1 - q)] = row[rr][q];
         }
     }
-    // PFIRST: the monomial equations as they stand after NP slides, column
-    // NP + j in register j (they are zero in the lambda columns, which the
-    // first NP steps eliminate without touching them)
-    auto load_monomial_rows = [&]() {
 #pragma unroll
-        for (int rr = T::R1; rr < ROWS; ++rr) {
-            const int e = eq_of(rr);
-            if (e < NS) {
+    for (int r = 0; r < T::QR; ++r) {
+        const int q = t + G * r;
+        if (q < NP) {
 #pragma unroll
-                for (int j = 0; j < NN; ++j) row[rr][j] = pblk[j * NP + (e - NN)];
-            } else {
-#pragma unroll
-                for (int j = 0; j < NN; ++j) row[rr][j] = 0.0;
-            }
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc) row[rr][NN + cc] = rhs[rr][cc];
+            for (int cc = 0; cc < R; ++cc) L.u[grp][T::MR0 + q * R + cc] = mrhs[r][cc];
         }
-    };
+    }
     __builtin_amdgcn_wave_barrier();
 
     // ---- Gauss-Jordan elimination with partial pivoting; the right-hand-side
@@ -614,9 +575,8 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         key = kk;
         crp = r;
     };
-    using RowsPhase1 = std::integral_constant<int, T::PFIRST ? T::R1 : ROWS>;
     using RowsAll = std::integral_constant<int, ROWS>;
-    candidate(RowsPhase1{});
+    candidate(RowsAll{});
     uint32_t kmax = group_max_u32<G>(key);
     int k = 0;
     // One step at static width WB >= WIDTH - k: slots below RF are updated
@@ -755,16 +715,38 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             }
         });
     };
-    if constexpr (T::PFIRST) {
-        run_steps(std::integral_constant<int, 0>{}, std::integral_constant<int, NP - 1>{}, RowsPhase1{}, RowsPhase1{});
-        load_monomial_rows();
-        step(std::integral_constant<int, WIDTH - (NP - 1)>{}, RowsPhase1{}, RowsAll{});
-        __builtin_amdgcn_wave_barrier();
-        ++k;
-        run_steps(std::integral_constant<int, NP>{}, std::integral_constant<int, NS>{}, RowsAll{}, RowsAll{});
-    } else {
-        run_steps(std::integral_constant<int, 0>{}, std::integral_constant<int, NS>{}, RowsAll{}, RowsAll{});
+    // the lambda columns, on the RBF equations
+    run_steps(std::integral_constant<int, 0>{}, std::integral_constant<int, NP>{}, RowsAll{}, RowsAll{});
+    // The monomial equations, as they stand after NP slides (column NP + j in
+    // register j: they are zero in the lambda columns, so those steps would
+    // not have changed them), into the free slots: the NP lambda pivot rows
+    // and the empty slots, at least NP in all.  The group's free slots are
+    // ranked slot by slot, lane by lane, and the slot of rank q takes monomial
+    // equation q.
+    {
+        int base = 0;
+#pragma unroll
+        for (int rr = 0; rr < ROWS; ++rr) {
+            const bool fr = alive[rr] == 0u;
+            const uint64_t bal = __builtin_amdgcn_ballot_w64(fr);
+            const uint64_t gm = G == 64 ? bal : (bal >> (grp * G)) & ((1ull << (G & 63)) - 1);
+            const int q = base + __builtin_popcountll(gm & ((1ull << t) - 1));
+            base += __builtin_popcountll(gm);
+            if (fr && q < NP) {
+#pragma unroll
+                for (int j = 0; j < NN; ++j) row[rr][j] = pblk[j * NP + q];
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) row[rr][NN + cc] = U[T::MR0 + q * R + cc];
+                alive[rr] = 0xFFFFFFFFu;
+                kp[rr] = NS;
+                pval[rr] = 1.0;
+            }
+        }
     }
+    candidate(RowsAll{});
+    kmax = group_max_u32<G>(key);
+    // the w columns, on the remaining RBF equations and the monomial equations
+    run_steps(std::integral_constant<int, NP>{}, std::integral_constant<int, NS>{}, RowsAll{}, RowsAll{});
     __builtin_amdgcn_wave_barrier();
 
     // ---- solution: after NS slides a row's registers 0..R-1 hold its
@@ -782,7 +764,7 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     uint32_t bad = 0;
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
-        const int u = kp[rr] - (T::PFIRST ? NP : 0);  // the w column it solved for
+        const int u = kp[rr] - NP;  // the w column it solved for
         if (u >= 0 && u < NN && live) {
             const int gu = L.g[grp][u];
             const long base = yk * NN + L.rank[grp][u];

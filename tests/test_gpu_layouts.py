@@ -25,16 +25,14 @@ TOL = 1e-10
 
 
 def layouts(n, polydeg):
-    """Lanes per stencil compiled for a shape: n + P <= 32 -> 16, 32;
-    <= 48 -> 32, 16, 64; <= 64 -> 32, 64; larger -> 64."""
+    """Lanes per stencil G compiled for a shape: a lane holds ceil(n / G) row
+    slots of n + P + 3 doubles (at three right-hand sides), a layout is
+    compiled where those take at most 200 registers and at least half of a
+    group's lanes hold a node, and 64 lanes when no layout qualifies.  Here:
+    n = 13 -> 16; 21, 31 -> 16, 32; 43, 57 -> 64."""
     ns = n + (polydeg + 1) * (polydeg + 2) // 2
-    if ns <= 32:
-        return [16, 32]
-    if ns <= 48:
-        return [32, 16, 64]
-    if ns <= 64:
-        return [32, 64]
-    return [64]
+    used = [g for g in (16, 32, 64) if -(-n // g) * (ns + 3) * 2 <= 200 and 2 * n > g]
+    return used or [64]
 
 
 CASES = [(p, d, n, g) for (p, d, n) in [(5, 2, 13), (3, 3, 21), (7, 4, 31), (7, 5, 43), (9, 6, 57)]
