@@ -92,6 +92,29 @@ __device__ __forceinline__ void row_bcast64x4(double v0, double v1, double v2, d
         : "v"(v0), "v"(v1), "v"(v2), "v"(v3), "i"(L0), "i"(L1), "i"(L2), "i"(L3));
 }
 
+// The sliding update of four columns in place with scalar pivot entries:
+// r0 <- a s0 + r1, r1 <- a s1 + r2, r2 <- a s2 + r3, r3 <- a s3 + r4, in
+// that order (each FMA reads its source before the next one overwrites it).
+// The scalar operands come from v_readlane, a VALU write of a scalar
+// register, which a VALU read may use only two wait states later; the
+// compiler does not see inside the asm, so the s_nop provides them.
+__device__ __forceinline__ void slide4_s(double &r0, double &r1, double &r2, double &r3, double r4, double a,
+                                         double s0, double s1, double s2, double s3)
+{
+    asm("s_nop 1\n\t"
+        "v_fma_f64 %0, %4, %5, %1\n\t"
+        "v_fma_f64 %1, %4, %6, %2\n\t"
+        "v_fma_f64 %2, %4, %7, %3\n\t"
+        "v_fma_f64 %3, %4, %8, %9"
+        : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(r3)
+        : "v"(a), "s"(s0), "s"(s1), "s"(s2), "s"(s3), "v"(r4));
+}
+
+__device__ __forceinline__ void slide1_s(double &r0, double r1, double a, double s0)
+{
+    asm("s_nop 1\n\tv_fma_f64 %0, %1, %2, %3" : "+v"(r0) : "v"(a), "s"(s0), "v"(r1));
+}
+
 template <int B, int NB, class F>
 __device__ __forceinline__ void for_blocks(F &&f)
 {
@@ -581,9 +604,67 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
     int k = 0;
     // One step at static width WB >= WIDTH - k: slots below RF are updated
     // (the winner is among them), slots below RC enter the next pivot search.
+    // 64 lanes (one stencil per wavefront): the pivot lane is the same for the
+    // whole wavefront, so every lane reads the pivot row straight from that
+    // lane's registers (v_readlane into scalar registers, which the FMAs take
+    // as operands) and the step makes no LDS round trip.
+    auto step64 = [&](auto wbc, auto rfc, auto rcc) {
+        constexpr int WB = decltype(wbc)::value;
+        constexpr int RF = decltype(rfc)::value;
+        const int v = 127 - (int)((uint32_t)__builtin_amdgcn_readfirstlane((int)kmax) & 127u);
+        const int wl = v & 63, psel = v >> 6;
+        // lane wl's x, in every lane
+        auto rl = [&](double x) {
+            const long long b = __double_as_longlong(x);
+            const int lo = __builtin_amdgcn_readlane((int)b, wl);
+            const int hi = __builtin_amdgcn_readlane((int)(b >> 32), wl);
+            return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+        };
+        for_blocks<0, RF>([&](auto rc) {
+            constexpr int rp_slot = decltype(rc)::value;
+            if (psel != rp_slot) return;  // uniform
+            const double rp = rl(crp);
+            const double p1 = rl(row[rp_slot][1]);
+            if (t == wl) {
+                alive[rp_slot] = 0u;
+                kp[rp_slot] = k;
+                pval[rp_slot] = row[rp_slot][0];
+                // a zero multiplier keeps the pivot row as it is
+                row[rp_slot][0] = 0.0;
+            }
+            double nl[ROWS];
+#pragma unroll
+            for (int rr = 0; rr < RF; ++rr) nl[rr] = -(row[rr][0] * rp);
+#pragma unroll
+            for (int rr = 0; rr < RF; ++rr) row[rr][0] = __builtin_fma(nl[rr], p1, row[rr][1]);
+            candidate(rcc);
+            kmax = group_max_u32<G>(key);
+            // columns j = 1 .. WB-2 take column j+1 of the pivot row, in fours
+            constexpr int NQ = (WB - 2) / 4;
+            for_blocks<0, NQ>([&](auto qc) {
+                constexpr int j = 1 + 4 * decltype(qc)::value;
+                const double s0 = rl(row[rp_slot][j + 1]), s1 = rl(row[rp_slot][j + 2]);
+                const double s2 = rl(row[rp_slot][j + 3]), s3 = rl(row[rp_slot][j + 4]);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr)
+                    slide4_s(row[rr][j], row[rr][j + 1], row[rr][j + 2], row[rr][j + 3], row[rr][j + 4], nl[rr], s0, s1,
+                             s2, s3);
+            });
+            for_blocks<1 + 4 * NQ, WB - 1>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const double sj = rl(row[rp_slot][j + 1]);
+#pragma unroll
+                for (int rr = 0; rr < RF; ++rr) slide1_s(row[rr][j], row[rr][j + 1], nl[rr], sj);
+            });
+        });
+    };
     auto step = [&](auto wbc, auto rfc, auto rcc) {
         constexpr int WB = decltype(wbc)::value;
         constexpr int RF = decltype(rfc)::value;
+        if constexpr (G == 64) {
+            step64(wbc, rfc, rcc);
+            return;
+        }
         const int v = 127 - (int)(kmax & 127u);
         const bool iam = t == v % G;
         const int psel = v / G;
