@@ -23,7 +23,8 @@
 // Layout and elimination follow the specialised kernel (stencil_lu.hpp): G
 // lanes per stencil, ROWS row slots of [A | B] per lane in registers, Gauss-
 // Jordan with implicit partial pivoting, the pivot row published through LDS
-// and spread by DPP broadcasts, the next pivot search overlapping each step.
+// and spread by DPP broadcasts (read with v_readlane at 64 lanes), the next
+// pivot search overlapping each step.
 // Column order: w (n), lambda (P), padding.  Matrix entries come from LDS
 // tables (scaled coordinates and the monomial block, stride P), so register
 // indices stay static while n and P vary at run time.
@@ -343,8 +344,61 @@ __global__ __launch_bounds__(64 * WPB, 1) void solve_gen_kernel(const SolveArgs 
     candidate();
     uint32_t kmax = group_max_u32<G>(key);
     int k = 0;
+    // 64 lanes (one stencil per wavefront): the pivot lane is wavefront-
+    // uniform and every lane reads the pivot row from its registers with
+    // v_readlane (as solve_lu_kernel does at 64 lanes); no LDS round trip
+    auto step64 = [&](auto wbc) {
+        constexpr int WB = decltype(wbc)::value;
+        const int v = 127 - (int)((uint32_t)__builtin_amdgcn_readfirstlane((int)kmax) & 127u);
+        const int wl = v & 63, psel = v >> 6;
+        auto rl = [&](double x) {
+            const long long b = __double_as_longlong(x);
+            const int lo = __builtin_amdgcn_readlane((int)b, wl);
+            const int hi = __builtin_amdgcn_readlane((int)(b >> 32), wl);
+            return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+        };
+        for_blocks<0, ROWS>([&](auto rc) {
+            constexpr int ps = decltype(rc)::value;
+            if (psel != ps) return;  // uniform
+            const double rp = rl(crp);
+            const double p1 = rl(row[ps][1]);
+            if (t == wl) {
+                alive[ps] = 0u;
+                kp[ps] = k;
+                pval[ps] = row[ps][0];
+                row[ps][0] = 0.0;  // a zero multiplier keeps the pivot row as it is
+            }
+            double nl[ROWS];
+#pragma unroll
+            for (int rr = 0; rr < ROWS; ++rr) nl[rr] = -(row[rr][0] * rp);
+#pragma unroll
+            for (int rr = 0; rr < ROWS; ++rr) row[rr][0] = __builtin_fma(nl[rr], p1, row[rr][1]);
+            candidate();
+            kmax = group_max_u32<G>(key);
+            constexpr int NQ = (WB - 2) / 4;
+            for_blocks<0, NQ>([&](auto qc) {
+                constexpr int j = 1 + 4 * decltype(qc)::value;
+                const double s0 = rl(row[ps][j + 1]), s1 = rl(row[ps][j + 2]);
+                const double s2 = rl(row[ps][j + 3]), s3 = rl(row[ps][j + 4]);
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr)
+                    slide4_s(row[rr][j], row[rr][j + 1], row[rr][j + 2], row[rr][j + 3], row[rr][j + 4], nl[rr], s0, s1,
+                             s2, s3);
+            });
+            for_blocks<1 + 4 * NQ, WB - 1>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const double sj = rl(row[ps][j + 1]);
+#pragma unroll
+                for (int rr = 0; rr < ROWS; ++rr) slide1_s(row[rr][j], row[rr][j + 1], nl[rr], sj);
+            });
+        });
+    };
     auto step = [&](auto wbc) {
         constexpr int WB = decltype(wbc)::value;
+        if constexpr (G == 64) {
+            step64(wbc);
+            return;
+        }
         const int v = 127 - (int)(kmax & 127u);
         const bool iam = t == v % G;
         const int psel = v / G;

@@ -5,24 +5,26 @@
 // Replaces hot loops 1 and 2 of the reference (src/generate_operator.jl:55-65
 // and :89-166, method 3 at :388-467; src/hyperviscosity_operator.jl:66-161).
 //
-// Layout.  A wavefront holds 64/G stencils; each lane of a group holds ROWS
-// row slots of its stencil's saddle-point system
+// Layout.  A wavefront holds 64/G stencils; each lane of a group holds
+// ROWS = ceil(n / G) row slots of its stencil's saddle-point system
 //     A = [Phi P; P^T 0],  Phi_ij = |xi_i - xi_j|^m,  P_iq = x^a y^b (xi_i),
-// each row extended by its right-hand-side entries, all in registers: the
-// RBF equations in the first slots, the monomial equations after them.
+// each row extended by its right-hand-side entries, all in registers.  The
+// lambda columns go first (highest degree first): the first NP steps run on
+// the RBF equations alone, then the monomial equations take the slots of the
+// NP lambda pivot rows (which only determine lambda) and of the empty slots.
 // Elimination: a DPP max over the group picks the pivot row, the pivot lane
 // publishes that row (and its reciprocal pivot) to LDS, and every lane
 // eliminates column k from all of its other rows, earlier pivot rows included;
 // each lane reads every 16th pivot entry and a DPP broadcast feeds ROWS FMAs
-// per column.  The pivot search for step k+1 runs as soon as column k+1 is
-// updated, overlapping the rest of step k.  Rows never move (the
+// per column.  With 64 lanes (one stencil per wavefront) the pivot lane is
+// wavefront-uniform and the lanes read the pivot row from its registers with
+// v_readlane instead.  The pivot search for step k+1 runs as soon as column
+// k+1 is updated, overlapping the rest of step k.  Rows never move (the
 // permutation stays implicit), and a column leaves the registers once it is
 // eliminated.  The layout updates every row slot at every step anyway, so
 // eliminating above the pivot (Gauss-Jordan) costs no extra instructions, and
 // it leaves each row with pivot * x in its right-hand-side registers: no back
-// substitution.  Where the monomial equations fill slots of their own, the
-// lambda columns go first (highest degree first) and those slots sit out the
-// first NP steps.  On the local systems the forward error matches an LU
+// substitution.  On the local systems the forward error matches an LU
 // solve's to a small factor (about 1e-12 at n = 31 and 4e-11 at n = 57
 // against a 40-digit solve).
 //
