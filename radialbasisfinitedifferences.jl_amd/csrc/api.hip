@@ -116,7 +116,7 @@ struct rbffd_context {
     // autotuner: (n, P, right-hand sides) -> lanes per stencil of the FP64 kernel
     std::map<std::tuple<int, int, int>, int> lu_group;
     // (n, P, right-hand sides) -> whether Y rows sharing a stencil solve grouped
-    std::map<std::tuple<int, int, int>, bool> group_rows;
+    std::map<std::tuple<int, int, int, bool>, bool> group_rows;
     int last_group = 0;  // lanes per stencil of the last solve launch
     // neighbour-search workspace of the single-set calls, and one per batch branch
     rbffd_arena knn_arena;
@@ -479,6 +479,25 @@ bool grouping_enabled()
     return !(env && env[0] == '0');
 }
 
+// Grouped tasks of a shape with a specialised kernel run in that kernel
+// (its one-slot layout) unless RBFFD_TUNED_GROUPS=0 sends them to the generic one.
+bool tuned_groups(int n, int P, int precision)
+{
+    const char *env = std::getenv("RBFFD_TUNED_GROUPS");
+    return precision == RBFFD_PREC_FP64 && rbffd::solve_tuned(n, P, 0) && !(env && env[0] == '0');
+}
+
+// Output rows per grouped task of the specialised kernel for nops operators:
+// every row of an X node in one task (one factorisation per X node, as the
+// reference) while that fits the widest right-hand-side block; rows past it
+// go to further tasks.
+int lu_rows_per_task(int nops, int maxrows)
+{
+    if (nops < 1 || maxrows <= 1) return 1;
+    const int ky = std::min(rbffd::kLuGrpMaxRhs / nops, maxrows);
+    return ky < 1 ? 1 : ky;
+}
+
 // Largest device-side workspace of the solves of systems above 128 equations.
 constexpr long kGenScratchBytes = 1L << 30;
 // Rows from which the choice between grouped and per-row solves is timed.
@@ -504,6 +523,7 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
     // ---- group the Y rows by their X node
     int ky = 1;
     int *counts = nullptr, *fill = nullptr;
+    bool grp_tuned = tuned_groups(n, P, precision);
     if (y2x && !rows && precision == RBFFD_PREC_FP64 && M > 1 && grouping_enabled()) {
         counts = (int *)w.get(S_GRP_COUNTS, sizeof(int) * (N + 1));
         fill = (int *)w.get(S_GRP_FILL, sizeof(int) * N);
@@ -512,7 +532,13 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
         rbffd::launch_fill_i32(flag, 1, 0, s);
         max_kernel<<<(unsigned)std::min<long>(2048, nblk(N)), 256, 0, s>>>(counts, N, flag);
         RBFFD_HIP(hipGetLastError());
-        ky = rbffd::gen_rows_per_task(n, P, cmax, read_flag(c, flag), (double)M / (double)N);
+        const int maxrows = read_flag(c, flag);
+        // the specialised kernel's blocks reach 24 columns: where an X node
+        // has more rows, calls below the tuning size keep one factorisation
+        // per X node in the generic kernel (blocks up to 72 columns)
+        if (grp_tuned && maxrows * cmax > rbffd::kLuGrpMaxRhs && M < kGroupTuneRows) grp_tuned = false;
+        ky = grp_tuned ? lu_rows_per_task(cmax, maxrows)
+                       : rbffd::gen_rows_per_task(n, P, cmax, maxrows, (double)M / (double)N);
     }
     const int *grp_x = nullptr, *grp_start = nullptr, *grp_count = nullptr, *ylist = nullptr, *ntd = nullptr;
     if (ky > 1) {
@@ -549,6 +575,7 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
     auto launch_all = [&](int kyv) {
         const bool grouped = kyv > 1;
         const bool tuned = !grouped && rbffd::solve_tuned(n, P, precision);
+        const bool grp_tuned_now = grouped && grp_tuned;
         // at most kMaxRhs right-hand sides per launch; more operators run in chunks
         for (int o0 = 0; o0 < nops; o0 += rbffd::kMaxRhs) {
             const int cnt = std::min(rbffd::kMaxRhs, nops - o0);
@@ -585,6 +612,7 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
                 a.grp_count = grp_count;
                 a.ylist = ylist;
                 a.ntasks_dev = ntd;
+                a.grp_tuned = grp_tuned_now ? 1 : 0;
             }
             a.scratch = mem;
             a.mem_ld = mem_ld;
@@ -600,6 +628,8 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
                 int g[rbffd::kMaxGroups];
                 rbffd::solve_groups(n, P, precision, g);
                 a.group = g[0];
+            } else if (grp_tuned_now) {
+                a.group = rbffd::lu_grp_lanes(n);
             } else {
                 a.group = rbffd::gen_lanes_used(n, P, a.gen_cols);
             }
@@ -608,16 +638,17 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
             RBFFD_HIP(hipGetLastError());
         }
     };
-    // A shape with a specialised kernel solves grouped tasks in the generic
-    // one, which can be slower than one specialised solve per row although it
-    // factors less (at 4 rows per X node and n = 31 it is).  The first large
-    // call per (shape, operators) times both and keeps the faster; smaller
-    // calls group.  RBFFD_GROUP_ROWS=1 forces grouping.
+    // A shape with a specialised kernel solves grouped tasks in that kernel's
+    // one-slot layout (the generic kernel with RBFFD_TUNED_GROUPS=0); with
+    // few rows per X node one solve per row in the default layout can still
+    // be faster.  The first large call per (shape, operators, kernel) times
+    // both and keeps the faster; smaller calls group.  RBFFD_GROUP_ROWS=1
+    // forces grouping.
     bool group = ky > 1;
     if (group && rbffd::solve_tuned(n, P, precision)) {
         const char *env = std::getenv("RBFFD_GROUP_ROWS");
         const bool forced = env && env[0] == '1';
-        auto key = std::make_tuple(n, P, cmax);
+        auto key = std::make_tuple(n, P, cmax, grp_tuned);
         auto it = c->group_rows.find(key);
         if (forced) {
             group = true;

@@ -91,6 +91,9 @@ struct SolveArgs {
     // sharing one stencil.  Tasks of several rows: task -> X node grp_x, its
     // rows ylist[grp_start .. grp_start + grp_count); NULL: one row per task.
     const int *grp_x, *grp_start, *grp_count, *ylist;
+    // grouped tasks of a shape with a specialised kernel: 1 = solve them in
+    // that kernel's one-slot layout (gen_cols <= kLuGrpMaxRhs), 0 = the generic one
+    int grp_tuned;
     const int *ntasks_dev;  // if set, tasks at or past *ntasks_dev do nothing
     int gen_cols;           // right-hand-side columns of the widest task
     // right-hand-side column cc: output row col_sub[cc] of the task, operator
@@ -140,9 +143,15 @@ void launch_group_build(const int *y2x, long M, long N, int ky, const int *count
 #define RBFFD_SHAPES(X) X(13, 6) X(20, 10) X(21, 10) X(31, 15) X(42, 21) X(43, 21) X(57, 28)
 #define RBFFD_DECLARE_SHAPE(nn, pp)                                                     \
     void launch_solve_##nn##_##pp(const SolveArgs &a, int precision, hipStream_t s); \
+    void launch_solve_grp_##nn##_##pp(const SolveArgs &a, hipStream_t s);            \
     void groups_##nn##_##pp(int precision, int *groups);
 RBFFD_SHAPES(RBFFD_DECLARE_SHAPE)
 #undef RBFFD_DECLARE_SHAPE
+// Grouped tasks in a specialised kernel: right-hand-side blocks of 6, 12 and
+// 24 columns, in the layout that holds the stencil's nodes in one row slot
+// per lane.
+constexpr int kLuGrpMaxRhs = 24;
+constexpr int lu_grp_lanes(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 // Generic kernel per padded system size NSB (R: right-hand-side columns
 // needed): gen_lanes(NSB) lanes per stencil, right-hand-side blocks of 3, 6 and
 // gen_rb(NSB) columns (several output rows per task where that exceeds 6).

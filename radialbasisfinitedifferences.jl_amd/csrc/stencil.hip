@@ -88,6 +88,13 @@ void launch_solve(const SolveArgs &a, int precision, hipStream_t s)
         RBFFD_SHAPES(RBFFD_CASE)
 #undef RBFFD_CASE
     }
+    if (a.grp_x && a.grp_tuned && solve_tuned(a.n, a.P, 0)) {
+        if (a.gen_cols > kLuGrpMaxRhs) throw Error{3, "grouped specialised solve: bad launch arguments"};
+#define RBFFD_CASE(nn, pp) \
+    if (a.n == nn && a.P == pp) { launch_solve_grp_##nn##_##pp(a, s); return; }
+        RBFFD_SHAPES(RBFFD_CASE)
+#undef RBFFD_CASE
+    }
     if (!solve_supported(a.n, a.P))
         throw Error{2, "no kernel for stencil size n=" + std::to_string(a.n) + " with " + std::to_string(a.P) +
                            " monomials (n must be in 1..128 and n + P at most 512)"};

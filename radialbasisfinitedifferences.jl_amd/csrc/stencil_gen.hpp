@@ -38,28 +38,6 @@
 
 namespace rbffd {
 
-// Bitwise OR over each aligned group of G lanes, in every lane of the group:
-// DPP inside 16-lane rows, then the lane swaps across rows and halves.
-template <int G>
-__device__ __forceinline__ uint64_t group_or_u64(uint64_t b)
-{
-    b = row_or_u64(b);
-    uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
-    if constexpr (G >= 32) {
-        const auto sl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-        const auto sh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-        lo = (uint32_t)sl[0] | (uint32_t)sl[1];
-        hi = (uint32_t)sh[0] | (uint32_t)sh[1];
-    }
-    if constexpr (G == 64) {
-        const auto sl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-        const auto sh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-        lo = (uint32_t)sl[0] | (uint32_t)sl[1];
-        hi = (uint32_t)sh[0] | (uint32_t)sh[1];
-    }
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // Exponents (a, b) of monomial q (x^a y^b, graded by total degree) for a q
 // known only at run time.
 __device__ __forceinline__ void mono_exponents(int q, int &ea, int &eb)

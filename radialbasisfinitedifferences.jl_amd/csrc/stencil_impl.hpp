@@ -405,6 +405,29 @@ void launch_lu(const SolveArgs &a, hipStream_t s)
     solve_lu_kernel<NN, NP, R, G, wpb, lu_min_waves<NN, NP, R, G>()><<<blocks, 64 * wpb, 0, s>>>(a);
 }
 
+// Grouped tasks (Y rows sharing a stencil, a.grp_x) in the specialised
+// kernel, one row slot per lane.
+template <int NN, int NP, int R>
+void launch_lu_grp(const SolveArgs &a, hipStream_t s)
+{
+    constexpr int G = lu_grp_lanes(NN);
+    constexpr int bytes = (int)(sizeof(LuLDS<NN, NP, R, G>) + sizeof(LuRows<R, 64 / G>));
+    constexpr int wpb = G == 16 ? 1 : lu_wpb(bytes);
+    constexpr int per_block = wpb * (64 / G);
+    const unsigned blocks = (unsigned)((a.ntasks + per_block - 1) / per_block);
+    solve_lu_kernel<NN, NP, R, G, wpb, lu_min_waves<NN, NP, R, G>(), true><<<blocks, 64 * wpb, 0, s>>>(a);
+}
+
+template <int NN, int NP>
+void launch_shape_grp(const SolveArgs &a, hipStream_t s)
+{
+    static_assert(kLuGrpMaxRhs == 24, "right-hand-side blocks");
+    if (a.gen_cols <= 6) launch_lu_grp<NN, NP, 6>(a, s);
+    else if (a.gen_cols <= 12) launch_lu_grp<NN, NP, 12>(a, s);
+    else if (a.gen_cols <= 24) launch_lu_grp<NN, NP, 24>(a, s);
+    else throw Error{2, "grouped tasks of at most 24 right-hand-side columns in a specialised kernel"};
+}
+
 // Lane-group sizes compiled for a stencil size: g0 is the default, the others
 // (0 = none) are candidates for the autotuner.  A lane holds ceil(n / G) row
 // slots of n + P + r doubles; a layout is compiled where those stay within
@@ -468,6 +491,7 @@ void shape_groups(int precision, int *groups)
     {                                                                                          \
         launch_shape<nn, pp>(a, precision, s);                                                 \
     }                                                                                          \
+    void launch_solve_grp_##nn##_##pp(const SolveArgs &a, hipStream_t s) { launch_shape_grp<nn, pp>(a, s); } \
     void groups_##nn##_##pp(int precision, int *groups) { shape_groups<nn, pp>(precision, groups); } \
     }
 

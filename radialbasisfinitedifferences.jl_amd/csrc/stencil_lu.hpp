@@ -168,6 +168,28 @@ __device__ __forceinline__ uint64_t row_or_u64(uint64_t b)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Bitwise OR over each aligned group of G lanes, in every lane of the group:
+// DPP inside 16-lane rows, then the lane swaps across rows and halves.
+template <int G>
+__device__ __forceinline__ uint64_t group_or_u64(uint64_t b)
+{
+    b = row_or_u64(b);
+    uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+    if constexpr (G >= 32) {
+        const auto sl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto sh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        lo = (uint32_t)sl[0] | (uint32_t)sl[1];
+        hi = (uint32_t)sh[0] | (uint32_t)sh[1];
+    }
+    if constexpr (G == 64) {
+        const auto sl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto sh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        lo = (uint32_t)sl[0] | (uint32_t)sl[1];
+        hi = (uint32_t)sh[0] | (uint32_t)sh[1];
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Maximum of non-negative doubles over each aligned group of G lanes (their
 // bit patterns order like unsigned integers): DPP inside 16-lane rows, then
 // lane swaps across rows and halves, as group_max_u32.
@@ -249,6 +271,10 @@ __device__ __forceinline__ PhsAt phs_at(int m, double vx, double vy)
 __device__ __forceinline__ double op_phs_fast(int op, int k, int m, double vx, double vy, double sx, double sy,
                                               const PhsAt &q)
 {
+    // no contraction: the value does not depend on how the caller's code
+    // around it lets the compiler fuse multiplies and adds (rows solved alone
+    // and rows solved in a group get the same bits)
+#pragma clang fp contract(off)
     if (op > OP_DXY && op != OP_LAP) return op_phs(op, k, m, vx, vy, sx, sy);
     // E uses the matrix entries' own formula: for collocated rows b = A e_c holds
     // bit for bit, so the solve returns the identity row to rounding
@@ -320,29 +346,57 @@ struct alignas(16) LuLDS {
     int rank[T::S][NN];  // rank of node i's global index inside the stencil
 };
 
-template <int NN, int NP, int R, int G, int WPB, int MINW>
+// Grouped tasks: the Y points of a task's output rows in the stencil frame,
+// and the rows themselves.
+template <int R, int S>
+struct LuRows {
+    double ys[S][R][2];
+    int yrow[S][R];
+};
+
+// GRP: a task is one stencil with the Y rows that use it (a.grp_x,
+// a.grp_start, a.grp_count, a.ylist); right-hand-side column cc belongs to
+// output row a.col_sub[cc] of the task and operator a.col_op[cc], and every
+// column sees the same operations wherever it sits, so a row's weights are
+// those of a task of its own, bit for bit.  Else one output row per task.
+template <int NN, int NP, int R, int G, int WPB, int MINW, bool GRP = false>
 __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArgs a)
 {
     using T = LuShape<NN, NP, R, G>;
     constexpr int NS = T::NS, ROWS = T::ROWS, WIDTH = T::WIDTH, S = T::S, NODES = T::NODES;
     static_assert(NS <= 128, "system size above 128");
+    static_assert(!GRP || R <= 64, "one status bit per output row of a task");
     __shared__ LuLDS<NN, NP, R, G> lds_all[WPB];
+    __shared__ LuRows<R, S> rows_all[GRP ? WPB : 1];
 
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int grp = lane / G;
     const int t = lane % G;
     auto &L = lds_all[w];
+    long ntasks = a.ntasks;
+    if (GRP && a.ntasks_dev) ntasks = min(ntasks, (long)*a.ntasks_dev);
     // every lane of the wavefront stays until the end (shuffles span the wave);
     // a group without a task works on a copy of task 0 and stores nothing
     const long task0 = ((long)blockIdx.x * WPB + w) * S;
-    if (task0 >= a.ntasks) return;  // whole wavefront
+    if (task0 >= ntasks) return;  // whole wavefront
     long task = task0 + grp;
-    const bool live = task < a.ntasks;
+    const bool live = task < ntasks;
     if (!live) task = task0;
 
-    const long yk = a.rows ? (long)a.rows[task] : task;
-    const long xs = a.y2x ? (long)a.y2x[yk] : yk;
+    long yk, xs, ybase = 0;
+    int ny = 1;
+    if constexpr (GRP) {
+        xs = a.grp_x[task];
+        ybase = a.grp_start[task];
+        ny = a.grp_count[task];
+        yk = a.ylist[ybase];
+    } else {
+        yk = a.rows ? (long)a.rows[task] : task;
+        xs = a.y2x ? (long)a.y2x[yk] : yk;
+    }
+    // right-hand-side columns in use (at most R: the launcher's choice)
+    const int ncols = GRP ? ny * a.nops : a.nops;
     const int m = a.m;
 
     // ---- stencil frame (src/scalestencil.jl:10-21, src/generate_operator.jl:100-134)
@@ -374,6 +428,16 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             sy = 1.0 / my;
             yx = (a.Y[2 * yk] - cx) * sx;
             yy = (a.Y[2 * yk + 1] - cy) * sy;
+        }
+        if constexpr (GRP) {
+            // the task's Y points in the stencil frame (src/generate_operator.jl:112-124);
+            // method 3 evaluates the polynomial right-hand side at the centre (:429-430)
+            for (int q = t; q < ny; q += G) {
+                const long yq = a.ylist[ybase + q];
+                rows_all[w].yrow[grp][q] = (int)yq;
+                rows_all[w].ys[grp][q][0] = a.method == 3 ? kEps : (a.Y[2 * yq] - cx) * sx;
+                rows_all[w].ys[grp][q][1] = a.method == 3 ? kEps : (a.Y[2 * yq + 1] - cy) * sy;
+            }
         }
 #pragma unroll
         for (int r = 0; r < NODES; ++r) {
@@ -470,23 +534,41 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
         if (e < NN) {
             const int i = e;
             const double xix = L.u[grp][T::XI0 + 2 * i], xiy = L.u[grp][T::XI0 + 2 * i + 1];
-            double ux, uy;
-            {
+            // u = Y - xi for the Y point at (px, py)
+            auto u_at = [&](double px, double py, double &ux, double &uy) {
 #pragma clang fp contract(off)
                 if (a.method == 3) {
                     ux = xix;
                     uy = xiy;
                 } else {
-                    ux = yx - xix;
-                    uy = yy - xiy;
+                    ux = px - xix;
+                    uy = py - xiy;
                     if (ux == 0.0) ux = kEps;
                     if (uy == 0.0) uy = kEps;
                 }
-            }
-            const PhsAt q = phs_at(m, ux, uy);
+            };
+            if constexpr (GRP) {
+                double ux = 0.0, uy = 0.0;
+                PhsAt q{};
 #pragma unroll
-            for (int cc = 0; cc < R; ++cc)
-                if (cc < a.nops) rhs[rr][cc] = op_phs_fast(a.op_kind[cc], a.op_k[cc], m, ux, uy, sx, sy, q);
+                for (int cc = 0; cc < R; ++cc) {
+                    if (cc < ncols) {
+                        if (a.col_op[cc] == 0) {  // the first operator of the next output row
+                            const int sub = a.col_sub[cc];
+                            u_at(rows_all[w].ys[grp][sub][0], rows_all[w].ys[grp][sub][1], ux, uy);
+                            q = phs_at(m, ux, uy);
+                        }
+                        rhs[rr][cc] = op_phs_fast(a.col_kind[cc], a.col_k[cc], m, ux, uy, sx, sy, q);
+                    }
+                }
+            } else {
+                double ux, uy;
+                u_at(yx, yy, ux, uy);
+                const PhsAt q = phs_at(m, ux, uy);
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc)
+                    if (cc < a.nops) rhs[rr][cc] = op_phs_fast(a.op_kind[cc], a.op_k[cc], m, ux, uy, sx, sy, q);
+            }
         }
     }
     double mrhs[T::QR][R];
@@ -500,8 +582,17 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
             while ((tt + 1) * (tt + 2) / 2 <= q) ++tt;
             const int ea = q - tt * (tt + 1) / 2, eb = tt - ea;
 #pragma unroll
-            for (int cc = 0; cc < R; ++cc)
-                if (cc < a.nops) mrhs[r][cc] = op_mono(a.op_kind[cc], a.op_k[cc], ea, eb, yx, yy, sx, sy);
+            for (int cc = 0; cc < R; ++cc) {
+                if constexpr (GRP) {
+                    if (cc < ncols) {
+                        const int sub = a.col_sub[cc];
+                        mrhs[r][cc] = op_mono(a.col_kind[cc], a.col_k[cc], ea, eb, rows_all[w].ys[grp][sub][0],
+                                              rows_all[w].ys[grp][sub][1], sx, sy);
+                    }
+                } else {
+                    if (cc < a.nops) mrhs[r][cc] = op_mono(a.op_kind[cc], a.op_k[cc], ea, eb, yx, yy, sx, sy);
+                }
+            }
         }
     }
 
@@ -844,6 +935,45 @@ __global__ __launch_bounds__(64 * WPB, MINW) void solve_lu_kernel(const SolveArg
 
     // ---- CSR output: the row that solved for node u writes it, at the rank
     // of u's global index inside the stencil (sorted columns)
+    if constexpr (GRP) {
+        // column cc to output row col_sub[cc], operator col_op[cc]; one
+        // non-finite bit per output row
+        uint64_t badm = 0;
+#pragma unroll
+        for (int rr = 0; rr < ROWS; ++rr) {
+            const int u = kp[rr] - NP;
+            if (u >= 0 && u < NN && live) {
+                const int gu = L.g[grp][u];
+                const int rku = L.rank[grp][u];
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) {
+                    if (cc < ncols) {
+                        const int sub = a.col_sub[cc], o = a.col_op[cc];
+                        const long base = (long)rows_all[w].yrow[grp][sub] * NN + rku;
+                        const double f = a.method == 3 ? 1.0 : op_post_scale(a.col_kind[cc], a.col_k[cc], sx, sy);
+                        const double wv = f * xv[rr][cc];
+                        if (!__builtin_isfinite(wv)) badm |= 1ull << sub;
+                        a.vals[o][base] = wv;
+                        if (o == 0 && a.colidx) a.colidx[base] = gu;
+                    }
+                }
+            }
+        }
+        bool singular = false;
+#pragma unroll
+        for (int rr = 0; rr < ROWS; ++rr) singular |= pval[rr] == 0.0;
+        const uint32_t sflag = group_max_u32<G>(singular ? 1u : 0u);
+        badm = group_or_u64<G>(badm);
+        if (a.status && live) {
+            for (int q = t; q < ny; q += G) {
+                const int yq = rows_all[w].yrow[grp][q];
+                const int st = sflag ? 1 : (((badm >> q) & 1u) ? 2 : 0);
+                if (a.status_accumulate) a.status[yq] = max(a.status[yq], st);
+                else a.status[yq] = st;
+            }
+        }
+        return;
+    }
     uint32_t bad = 0;
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {

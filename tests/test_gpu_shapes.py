@@ -163,3 +163,41 @@ def test_one_factorisation_per_x_node(ctx, O, p, polydeg, n, fitted_nodes):
     st = st_d.get()
     rows = _sample(len(Y), 300, n)
     _check(O, X, Y, st, y2x, p, polydeg, ops, 0, 1, col.get(), [v.get() for v in vals], rows)
+
+
+@pytest.mark.parametrize("p,polydeg,n", [(3, 3, 20), (7, 4, 31), (3, 5, 42)])
+def test_grouped_specialised_equal_single_rows(ctx, O, p, polydeg, n, fitted_nodes, monkeypatch):
+    """A shape with a specialised kernel solves Y rows grouped by X node in
+    that kernel's one-slot layout (16, 32 or 64 lanes for n <= 16, 32, 64).
+    Each right-hand-side column sees the operations it would see in a task
+    of its own in that layout, so the grouped weights equal one-row-per-task
+    weights in the same layout bit for bit; against the quad oracle they meet
+    the bar, and the generic kernel's grouped weights agree with them to
+    rounding."""
+    X, Y = fitted_nodes
+    st, _ = O.knn(X, X, n)
+    y2x = O.knn(X, Y, 1)[0][:, 0]
+    ops = ["Dx", "Dy", "Lap"]
+    g = 16 if n <= 16 else (32 if n <= 32 else 64)
+    print(f"grouped specialised vs one row per task, n = {n}, {g} lanes", flush=True)
+    monkeypatch.setenv("RBFFD_GROUP_ROWS", "1")
+    c1, v1, s1 = _gpu_weights(ctx, X, Y, st, y2x, p, polydeg, ops, 0, 1)
+    assert not s1.any()
+    monkeypatch.setenv("RBFFD_GROUP_ROWS", "0")
+    monkeypatch.setenv("RBFFD_LU_GROUP", str(g))
+    c2, v2, s2 = _gpu_weights(ctx, X, Y, st, y2x, p, polydeg, ops, 0, 1)
+    assert np.array_equal(c1, c2) and np.array_equal(s1, s2)
+    for name, a, b in zip(ops, v1, v2):
+        nd = int((a != b).any(axis=-1).sum()) if a.ndim > 1 else int((a != b).sum())
+        print(f"  {name}: rows differing {nd}, max rel {rowrel(a.reshape(len(Y), -1), b.reshape(len(Y), -1)).max():.3e}", flush=True)
+    for a, b in zip(v1, v2):
+        assert np.array_equal(a, b)
+    monkeypatch.delenv("RBFFD_LU_GROUP")
+    monkeypatch.setenv("RBFFD_GROUP_ROWS", "1")
+    monkeypatch.setenv("RBFFD_TUNED_GROUPS", "0")
+    c3, v3, s3 = _gpu_weights(ctx, X, Y, st, y2x, p, polydeg, ops, 0, 1)
+    assert np.array_equal(c1, c3) and not s3.any()
+    for a, b in zip(v1, v3):
+        assert rowrel(a, b).max() < 1e-9
+    rows = _sample(len(Y), 300, n)
+    _check(O, X, Y, st, y2x, p, polydeg, ops, 0, 1, c1, v1, rows)
