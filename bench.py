@@ -489,9 +489,10 @@ def extra_advdiff(rbffd_amd, ctx, eng, no_cpu):
 def extra_ydistinct(rbffd_amd, ctx, eng, no_cpu):
     """Y distinct from X: X the 1024^2 workload nodes (r^7, degree 4, n = 31),
     Y a 2047^2 jittered grid (4,190,209 rows, about 4 per X node), operators
-    Dx, Dy, Lap: the library's own choice (timed on the first large call), the
-    rows of one X node solved against one factorisation of its system, and
-    one factorisation per row."""
+    Dx, Dy, Lap: the library's own choice (timed on the first large call),
+    rows grouped by X node (each task's rows solved against one factorisation
+    of its system; an X node with more rows than a task holds takes several),
+    and one factorisation per row."""
     from rbffd_amd.nodes import config_nodes, jittered_grid
 
     log("extra: Y distinct from X, 1M X nodes, 4.2M Y rows")
@@ -506,7 +507,7 @@ def extra_ydistinct(rbffd_amd, ctx, eng, no_cpu):
     old = os.environ.get("RBFFD_GROUP_ROWS")
     try:
         outs = {}
-        for tag, flag in (("default_choice", None), ("one_factorisation_per_x_node", "1"),
+        for tag, flag in (("default_choice", None), ("grouped_by_x_node", "1"),
                           ("one_factorisation_per_row", "0")):
             if flag is None:
                 os.environ.pop("RBFFD_GROUP_ROWS", None)
@@ -519,7 +520,7 @@ def extra_ydistinct(rbffd_amd, ctx, eng, no_cpu):
                         "factorizations": stats["factorizations"],
                         "rows_per_s_solve": M / (stats["ms_solve"] * 1e-3)}
             outs[tag] = [v.get() for v in vals]
-        a, b = outs["one_factorisation_per_x_node"], outs["one_factorisation_per_row"]
+        a, b = outs["grouped_by_x_node"], outs["one_factorisation_per_row"]
         res["max_row_dev_between_the_two"] = max(row_errors(x, y)[0] for x, y in zip(a, b))
     finally:
         if old is None:

@@ -487,14 +487,27 @@ bool tuned_groups(int n, int P, int precision)
     return precision == RBFFD_PREC_FP64 && rbffd::solve_tuned(n, P, 0) && !(env && env[0] == '0');
 }
 
-// Output rows per grouped task of the specialised kernel for nops operators:
-// every row of an X node in one task (one factorisation per X node, as the
-// reference) while that fits the widest right-hand-side block; rows past it
-// go to further tasks.
-int lu_rows_per_task(int nops, int maxrows)
+// Output rows per grouped task of the specialised kernel for nops operators.
+// once: every row of an X node in one task (one factorisation per X node, as
+// the reference) while that fits the widest right-hand-side block.  Else the
+// narrowest block that holds the mean rows per X node (narrow rows cost less
+// per step than the factorisations they add: 4M rows over 1M X nodes at
+// n = 31 take 17.3 ms in tasks of 4 rows, 19.5 ms in tasks of 8, 24.2 ms in
+// tasks of 2).  Rows past ky go to further tasks.
+int lu_rows_per_task(int nops, int maxrows, double mean_rows, bool once)
 {
     if (nops < 1 || maxrows <= 1) return 1;
-    const int ky = std::min(rbffd::kLuGrpMaxRhs / nops, maxrows);
+    int ky = rbffd::kLuGrpMaxRhs / nops;
+    if (!once) {
+        const int want = (int)(mean_rows + 0.999);
+        for (int b = 6; b <= rbffd::kLuGrpMaxRhs; b *= 2) {
+            if (b / nops >= want) {
+                ky = b / nops;
+                break;
+            }
+        }
+    }
+    ky = std::min(ky, maxrows);
     return ky < 1 ? 1 : ky;
 }
 
@@ -533,11 +546,12 @@ long solves_enqueue(rbffd_context *c, Workspace &w, long N, const double *X, con
         max_kernel<<<(unsigned)std::min<long>(2048, nblk(N)), 256, 0, s>>>(counts, N, flag);
         RBFFD_HIP(hipGetLastError());
         const int maxrows = read_flag(c, flag);
-        // the specialised kernel's blocks reach 24 columns: where an X node
-        // has more rows, calls below the tuning size keep one factorisation
-        // per X node in the generic kernel (blocks up to 72 columns)
-        if (grp_tuned && maxrows * cmax > rbffd::kLuGrpMaxRhs && M < kGroupTuneRows) grp_tuned = false;
-        ky = grp_tuned ? lu_rows_per_task(cmax, maxrows)
+        // calls below the tuning size keep one factorisation per X node: in
+        // the specialised kernel while an X node's rows fit its 24-column
+        // block, else in the generic kernel (blocks up to 72 columns)
+        const bool small = M < kGroupTuneRows;
+        if (grp_tuned && maxrows * cmax > rbffd::kLuGrpMaxRhs && small) grp_tuned = false;
+        ky = grp_tuned ? lu_rows_per_task(cmax, maxrows, (double)M / (double)N, small)
                        : rbffd::gen_rows_per_task(n, P, cmax, maxrows, (double)M / (double)N);
     }
     const int *grp_x = nullptr, *grp_start = nullptr, *grp_count = nullptr, *ylist = nullptr, *ntd = nullptr;
