@@ -201,3 +201,36 @@ def test_grouped_specialised_equal_single_rows(ctx, O, p, polydeg, n, fitted_nod
         assert rowrel(a, b).max() < 1e-9
     rows = _sample(len(Y), 300, n)
     _check(O, X, Y, st, y2x, p, polydeg, ops, 0, 1, c1, v1, rows)
+
+
+def test_grouped_specialised_operator_chunks(ctx, O, fitted_nodes, monkeypatch):
+    """Nine operators run in chunks of six right-hand sides; with at most four
+    Y rows per X node every chunk fits the specialised kernel's 24-column
+    block (one factorisation per X node per chunk), and the grouped weights
+    equal one-row-per-task weights in the same layout bit for bit (n = 31,
+    degree 4, Dxk/Dyk at k = 3)."""
+    X, Y = fitted_nodes
+    n, p, polydeg = 31, 7, 4
+    st, _ = O.knn(X, X, n)
+    y2x_all = O.knn(X, Y, 1)[0][:, 0]
+    # at most four rows per X node (the fixture has up to five)
+    keep = np.ones(len(Y), bool)
+    seen = {}
+    for i, x in enumerate(y2x_all):
+        seen[x] = seen.get(x, 0) + 1
+        keep[i] = seen[x] <= 4
+    Y = np.ascontiguousarray(Y[keep])
+    y2x = y2x_all[keep]
+    ops = ["E", "Dx", "Dy", "Dxx", "Dyy", "Dxy", "Lap", "Dxk", "Dyk"]
+    print(f"grouped specialised, nine operators, {len(Y)} rows", flush=True)
+    monkeypatch.setenv("RBFFD_GROUP_ROWS", "1")
+    c1, v1, s1 = _gpu_weights(ctx, X, Y, st, y2x, p, polydeg, ops, 3, 1)
+    assert not s1.any()
+    monkeypatch.setenv("RBFFD_GROUP_ROWS", "0")
+    monkeypatch.setenv("RBFFD_LU_GROUP", "32")
+    c2, v2, s2 = _gpu_weights(ctx, X, Y, st, y2x, p, polydeg, ops, 3, 1)
+    assert np.array_equal(c1, c2) and np.array_equal(s1, s2)
+    for name, a, b in zip(ops, v1, v2):
+        assert np.array_equal(a, b), name
+    rows = _sample(len(Y), 200, 9)
+    _check(O, X, Y, st, y2x, p, polydeg, ops, 3, 1, c1, v1, rows)
