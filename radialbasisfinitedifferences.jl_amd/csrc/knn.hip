@@ -309,6 +309,17 @@ __global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__rest
     }
 }
 
+// kv = min(kv, c), c = max(kv, c): two instructions.  __builtin_fmin/fmax
+// first re-quiet both operands for the IEEE mode kernels run in, a third
+// instruction a slot; no NaN reaches this chain (a NaN distance fails the
+// d < kv test in front of it), so the plain instructions give the same values.
+__device__ __forceinline__ void minmax_slot(double &kv, double &c)
+{
+    double hi;
+    asm("v_max_f64 %0, %1, %2\n\tv_min_f64 %1, %1, %2" : "=&v"(hi), "+v"(kv) : "v"(c));
+    c = hi;
+}
+
 // Two-phase exact selection for larger k.  Phase 1 keeps only the KMAX
 // smallest squared distances, without their indices, in a min/max exchange
 // chain (two instructions a slot where the insertion needs about nine), which
@@ -318,7 +329,23 @@ __global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__rest
 // candidates have d <= tau; any other query (an exact distance tie, fewer than
 // k visible nodes, non-finite points) goes to `redo`, which the insertion
 // kernel then recomputes with the (d^2, index) order.
-template <int KMAX>
+//
+// With STAGE, phase 2 places each candidate's cell-order position in an LDS
+// row per thread (blockDim.x * k ints of dynamic shared memory) and each
+// wavefront then writes its 64 rows out together, consecutive lanes on
+// consecutive entries of a row, looking up the node index (and recomputing
+// d^2 the same way when it is asked for): stores made straight from phase 2
+// put 4 bytes into a different row per lane, spread over the whole cell walk,
+// and the partly written lines cost about 2.7x the output size in HBM writes.
+// Up to KMAX = 32 (and below 2^26 points) the rows are staged in the order
+// found and ranked after the walk, when every lane has its k candidates:
+// during the walk about a third of the lanes accept a given candidate, and
+// the rank costs the wavefront the same either way.  The rank is kept in bits
+// 26..30 of the staged entry, since storing each entry at its rank straight to
+// HBM from the owning lane is as scattered as storing during the walk.  Rows
+// sent to `redo` are not written here; the redo kernels write all k entries of
+// those.
+template <int KMAX, bool STAGE>
 __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__restrict__ gp,
                                                          const int *__restrict__ cell_start,
                                                          const double *__restrict__ cell_xy,
@@ -329,124 +356,208 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
                                                          const int *__restrict__ qorder, long M, int k,
                                                          int *__restrict__ idx_out, double *__restrict__ d2_out,
                                                          int *__restrict__ redo, double *__restrict__ redo_tau,
-                                                         int *__restrict__ redo_count)
+                                                         int *__restrict__ redo_count, int pack)
 {
 #pragma clang fp contract(off)
-    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= M) return;
-    const GridParams g = *gp;
-    const long q = qorder ? (long)qorder[t] : t;
-    if (!g.valid) {
-        const int slot = atomicAdd(redo_count, 1);
-        redo[slot] = (int)q;
-        redo_tau[slot] = INFINITY;
-        return;
-    }
-    const double qx = Q[2 * q], qy = Q[2 * q + 1];
-    const int qg = (cell_grp && qgroup) ? qgroup[q] : -1;
-
-    // kept values ascending; slots [0, KMAX-k) hold a sentinel below any distance
-    double kv[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j) kv[j] = j < KMAX - k ? -1.0 : INFINITY;
-
-    const double fx = (qx - g.x0) / g.csx, fy = (qy - g.y0) / g.csy;
-    const int qcx = (int)fmin(fmax(floor(fx), -2.0), (double)g.gx + 1.0);
-    const int qcy = (int)fmin(fmax(floor(fy), -2.0), (double)g.gy + 1.0);
-    const int rlimit = g.gx + g.gy + 4;
-
-    // the cells of ring r in a fixed order; f(cell, lower bound on d^2)
-    auto ring = [&](int r, auto &&f) {
-        const int ncell = r == 0 ? 1 : 8 * r;
-        for (int tc = 0; tc < ncell; ++tc) {
-            int dxc = 0, dyc = 0;
-            if (r > 0) {
-                int side = tc / (2 * r), off = tc - side * 2 * r;
-                if (side == 0) { dxc = -r + off; dyc = -r; }
-                else if (side == 1) { dxc = r; dyc = -r + off; }
-                else if (side == 2) { dxc = r - off; dyc = r; }
-                else { dxc = -r; dyc = r - off; }
-            }
-            const int cx = qcx + dxc, cy = qcy + dyc;
-            if (cx < 0 || cx >= g.gx || cy < 0 || cy >= g.gy) continue;
-            const double rx0 = g.x0 + cx * g.csx, rx1 = rx0 + g.csx;
-            const double ry0 = g.y0 + cy * g.csy, ry1 = ry0 + g.csy;
-            const double ddx = qx < rx0 ? rx0 - qx : (qx > rx1 ? qx - rx1 : 0.0);
-            const double ddy = qy < ry0 ? ry0 - qy : (qy > ry1 ? qy - ry1 : 0.0);
-            f(cy * g.gx + cx, (ddx * ddx + ddy * ddy) * (1.0 - 1e-9));
+    extern __shared__ int srow[];  // STAGE: [blockDim.x][k]
+    __shared__ int srow_q[STAGE ? 256 : 1];
+    const long t0 = blockIdx.x * (long)blockDim.x;
+    const long t = t0 + threadIdx.x;
+    int *const mine = STAGE ? srow + threadIdx.x * k : nullptr;
+    // rank after the walk: k <= 32 and cell positions below 2^26 (pack), so a
+    // staged entry holds its position and its rank
+    constexpr bool DENSE_OK = STAGE && KMAX <= 32;
+    const bool dense = DENSE_OK && pack;
+    // the query's row when every entry of it is placed, else -1
+    const long qdone = [&]() -> long {
+        if (t >= M) return -1;
+        const GridParams g = *gp;
+        const long q = qorder ? (long)qorder[t] : t;
+        if (!g.valid) {
+            const int slot = atomicAdd(redo_count, 1);
+            redo[slot] = (int)q;
+            redo_tau[slot] = INFINITY;
+            return -1;
         }
-    };
-    // squared distance to candidate p, or -1 when the query does not see it
-    auto dist = [&](int p) {
-        if (qg >= 0) {
-            const int h = cell_grp[p];
-            if (h != 0 && h != qg) return -1.0;
-        }
-        const double dx = cell_xy[2 * (long)p] - qx, dy = cell_xy[2 * (long)p + 1] - qy;
-        const double dd = dx * dx;
-        const double ee = dy * dy;
-        return dd + ee;
-    };
+        const double qx = Q[2 * q], qy = Q[2 * q + 1];
+        const int qg = (cell_grp && qgroup) ? qgroup[q] : -1;
 
-    // ---- phase 1: the KMAX smallest values
-    int rlast = 0;
-    for (int r = 0; r <= rlimit; ++r) {
-        rlast = r;
-        ring(r, [&](int c, double md) {
-            if (md > kv[KMAX - 1]) return;
-            const int p1 = cell_start[c + 1];
-            for (int p = cell_start[c]; p < p1; ++p) {
-                const double d = dist(p);
-                if (d >= 0.0 && d < kv[KMAX - 1]) {
-                    double cv = d;
+        // kept values ascending; slots [0, KMAX-k) hold a sentinel below any distance
+        double kv[KMAX];
 #pragma unroll
-                    for (int j = 0; j < KMAX; ++j) {
-                        const double lo = __builtin_fmin(cv, kv[j]);
-                        cv = __builtin_fmax(cv, kv[j]);
-                        kv[j] = lo;
-                    }
+        for (int j = 0; j < KMAX; ++j) kv[j] = j < KMAX - k ? -1.0 : INFINITY;
+
+        const double fx = (qx - g.x0) / g.csx, fy = (qy - g.y0) / g.csy;
+        const int qcx = (int)fmin(fmax(floor(fx), -2.0), (double)g.gx + 1.0);
+        const int qcy = (int)fmin(fmax(floor(fy), -2.0), (double)g.gy + 1.0);
+        const int rlimit = g.gx + g.gy + 4;
+
+        // the cells of ring r in a fixed order; f(cell, lower bound on d^2)
+        auto ring = [&](int r, auto &&f) {
+            const int ncell = r == 0 ? 1 : 8 * r;
+            for (int tc = 0; tc < ncell; ++tc) {
+                int dxc = 0, dyc = 0;
+                if (r > 0) {
+                    int side = tc / (2 * r), off = tc - side * 2 * r;
+                    if (side == 0) { dxc = -r + off; dyc = -r; }
+                    else if (side == 1) { dxc = r; dyc = -r + off; }
+                    else if (side == 2) { dxc = r - off; dyc = r; }
+                    else { dxc = -r; dyc = r - off; }
                 }
+                const int cx = qcx + dxc, cy = qcy + dyc;
+                if (cx < 0 || cx >= g.gx || cy < 0 || cy >= g.gy) continue;
+                const double rx0 = g.x0 + cx * g.csx, rx1 = rx0 + g.csx;
+                const double ry0 = g.y0 + cy * g.csy, ry1 = ry0 + g.csy;
+                // the gap to the cell per axis (0 inside), without branches
+                const double ddx = fmax(fmax(rx0 - qx, qx - rx1), 0.0);
+                const double ddy = fmax(fmax(ry0 - qy, qy - ry1), 0.0);
+                f(cy * g.gx + cx, (ddx * ddx + ddy * ddy) * (1.0 - 1e-9));
             }
-        });
-        const double bx0 = qx - (g.x0 + (double)(qcx - r) * g.csx);
-        const double bx1 = (g.x0 + (double)(qcx + r + 1) * g.csx) - qx;
-        const double by0 = qy - (g.y0 + (double)(qcy - r) * g.csy);
-        const double by1 = (g.y0 + (double)(qcy + r + 1) * g.csy) - qy;
-        const double b = fmin(fmin(bx0, bx1), fmin(by0, by1));
-        if (b > 0.0 && b * b * (1.0 - 1e-9) > kv[KMAX - 1]) break;
-        if (qcx - r <= 0 && qcy - r <= 0 && qcx + r >= g.gx - 1 && qcy + r >= g.gy - 1) break;
-    }
+        };
+        // squared distance to candidate p, or -1 when the query does not see it
+        auto d2of = [&](int p) {
+            const double dx = cell_xy[2 * (long)p] - qx, dy = cell_xy[2 * (long)p + 1] - qy;
+            const double dd = dx * dx;
+            const double ee = dy * dy;
+            return dd + ee;
+        };
+        auto dist = [&](int p) {
+            if (qg >= 0) {
+                const int h = cell_grp[p];
+                if (h != 0 && h != qg) return -1.0;
+            }
+            return d2of(p);
+        };
 
-    // ---- phase 2: every candidate with d <= tau at its rank
-    const double tau = kv[KMAX - 1];
-    bool tie = !(tau < INFINITY);
-#pragma unroll
-    for (int j = 1; j < KMAX; ++j)
-        if (j > KMAX - k) tie |= kv[j] == kv[j - 1];
-    int nsel = 0;
-    if (!tie) {
-        for (int r = 0; r <= rlast; ++r) {
+        // ---- phase 1: the KMAX smallest values
+        int rlast = 0;
+        for (int r = 0; r <= rlimit; ++r) {
+            rlast = r;
             ring(r, [&](int c, double md) {
-                if (md > tau) return;
+                if (md > kv[KMAX - 1]) return;
                 const int p1 = cell_start[c + 1];
                 for (int p = cell_start[c]; p < p1; ++p) {
                     const double d = dist(p);
-                    if (d >= 0.0 && d <= tau) {
-                        int pos = -(KMAX - k);
+                    if (d >= 0.0 && d < kv[KMAX - 1]) {
+                        double cv = d;
 #pragma unroll
-                        for (int j = 0; j < KMAX; ++j) pos += kv[j] < d ? 1 : 0;
-                        idx_out[q * k + pos] = cell_idx[p];
-                        if (d2_out) d2_out[q * k + pos] = d;
-                        ++nsel;
+                        for (int j = 0; j < KMAX; ++j) minmax_slot(kv[j], cv);
                     }
                 }
             });
+            const double bx0 = qx - (g.x0 + (double)(qcx - r) * g.csx);
+            const double bx1 = (g.x0 + (double)(qcx + r + 1) * g.csx) - qx;
+            const double by0 = qy - (g.y0 + (double)(qcy - r) * g.csy);
+            const double by1 = (g.y0 + (double)(qcy + r + 1) * g.csy) - qy;
+            const double b = fmin(fmin(bx0, bx1), fmin(by0, by1));
+            if (b > 0.0 && b * b * (1.0 - 1e-9) > kv[KMAX - 1]) break;
+            if (qcx - r <= 0 && qcy - r <= 0 && qcx + r >= g.gx - 1 && qcy + r >= g.gy - 1) break;
         }
-    }
-    if (tie || nsel != k) {
-        const int slot = atomicAdd(redo_count, 1);
-        redo[slot] = (int)q;
-        redo_tau[slot] = tie ? tau : (nsel > k ? tau : INFINITY);
+
+        // ---- phase 2: every candidate with d <= tau, at its rank
+        const double tau = kv[KMAX - 1];
+        bool tie = !(tau < INFINITY);
+#pragma unroll
+        for (int j = 1; j < KMAX; ++j)
+            if (j > KMAX - k) tie |= kv[j] == kv[j - 1];
+        // rank of d among the kept values: a compare and an add with carry a
+        // slot, kept apart from the address arithmetic, which the compiler
+        // would otherwise fold into every term
+        auto rank = [&](double d) {
+            int pos = 0;
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j) pos += kv[j] < d ? 1 : 0;
+            asm volatile("" : "+v"(pos));
+            return pos - (KMAX - k);
+        };
+        int nsel = 0;
+        if (!tie) {
+            for (int r = 0; r <= rlast; ++r) {
+                ring(r, [&](int c, double md) {
+                    if (md > tau) return;
+                    const int p1 = cell_start[c + 1];
+                    for (int p = cell_start[c]; p < p1; ++p) {
+                        const double d = dist(p);
+                        if (d >= 0.0 && d <= tau) {
+                            if (dense) {
+                                if (nsel < k) mine[nsel] = p;
+                            } else {
+                                const int pos = rank(d);
+                                if (STAGE) {
+                                    mine[pos] = p;
+                                } else {
+                                    idx_out[q * k + pos] = cell_idx[p];
+                                    if (d2_out) d2_out[q * k + pos] = d;
+                                }
+                            }
+                            ++nsel;
+                        }
+                    }
+                });
+            }
+        }
+        if (tie || nsel != k) {
+            const int slot = atomicAdd(redo_count, 1);
+            redo[slot] = (int)q;
+            redo_tau[slot] = tie ? tau : (nsel > k ? tau : INFINITY);
+            return -1;
+        }
+        if constexpr (DENSE_OK) {
+            if (dense) {
+                // the k candidates, staged in the order found, ranked with
+                // every lane of the wavefront at work (four at a time, so their
+                // coordinate loads overlap); the rank goes to bits 26..30
+                int j = 0;
+                for (; j + 4 <= k; j += 4) {
+                    int p[4];
+                    double d[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) p[u] = mine[j + u];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) d[u] = d2of(p[u]);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) mine[j + u] = p[u] | rank(d[u]) << 26;
+                }
+                for (; j < k; ++j) mine[j] = mine[j] | rank(d2of(mine[j])) << 26;
+            }
+        }
+        return q;
+    }();
+    if (!STAGE) return;
+
+    // ---- the wavefront's rows, written out entry by entry (a wavefront reads
+    // only the rows its own lanes staged, so no other wavefront is waited for)
+    const int lane = (int)threadIdx.x & 63, wbase = (int)threadIdx.x & ~63;
+    srow_q[threadIdx.x] = (int)qdone;
+    __builtin_amdgcn_wave_barrier();
+    const long w0 = t0 + wbase;
+    const int nw = (int)(M - w0 < 64 ? M - w0 : 64);
+    const int *const wrow = srow + wbase * k;
+    const int step_r = 64 / k, step_p = 64 - step_r * k;
+    int r = lane / k, pos = lane - r * k;
+    for (int e = lane; e < nw * k; e += 64) {
+        const int qq = srow_q[wbase + r];
+        if (qq >= 0) {
+            int p = wrow[e], at = pos;
+            if (dense) {
+                at = p >> 26;
+                p &= (1 << 26) - 1;
+            }
+            idx_out[(long)qq * k + at] = cell_idx[p];
+            if (d2_out) {
+                const double dx = cell_xy[2 * (long)p] - Q[2 * (long)qq];
+                const double dy = cell_xy[2 * (long)p + 1] - Q[2 * (long)qq + 1];
+                const double dd = dx * dx;
+                const double ee = dy * dy;
+                d2_out[(long)qq * k + at] = dd + ee;
+            }
+        }
+        r += step_r;
+        pos += step_p;
+        if (pos >= k) {
+            pos -= k;
+            ++r;
+        }
     }
 }
 
@@ -560,13 +671,17 @@ __global__ __launch_bounds__(64 * kTieWaves) void knn_tie_kernel(
 
 template <int KMAX>
 static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
-                         const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
+                         long N, const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
                          double *d2, int *redo, int *redo_count, const KnnRedo &rw, hipStream_t s)
 {
     const unsigned blocks = (unsigned)((M + 255) / 256);
     if (KMAX >= 16 && redo) {
-        knn_select_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2,
-                                                        redo, rw.tau, redo_count);
+        // rows staged in LDS up to 64 KiB a block (k <= 64)
+        constexpr bool stage = KMAX <= 64;
+        const size_t lds = stage ? sizeof(int) * 256 * (size_t)k : 0;
+        knn_select_kernel<KMAX, stage><<<blocks, 256, lds, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k,
+                                                                 idx, d2, redo, rw.tau, redo_count,
+                                                                 N <= (1L << 26) ? 1 : 0);
         // at most 16384 wavefronts walk the tied queries in turn
         const unsigned tblocks = (unsigned)std::max(1L, std::min<long>((M + kTieWaves - 1) / kTieWaves, 4096));
         knn_tie_kernel<<<tblocks, 64 * kTieWaves, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, redo, rw.tau,
@@ -580,7 +695,7 @@ static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, 
 }
 
 void launch_knn_query(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
-                      const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
+                      long N, const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
                       double *d2, int *redo, int *redo_count, const KnnRedo &rw, hipStream_t s)
 {
     if (M <= 0) return;
@@ -589,7 +704,7 @@ void launch_knn_query(const GridParams *g, const int *cs, const double *cxy, con
         launch_fill_i32(rw.count2, 1, 0, s);
     }
 #define RBFFD_KNN(KM) \
-    knn_launch_t<KM>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k, idx, d2, redo, redo_count, rw, s)
+    knn_launch_t<KM>(g, cs, cxy, cidx, cgrp, N, Q, qgroup, qorder, M, k, idx, d2, redo, redo_count, rw, s)
     if (k <= 1) RBFFD_KNN(1);
     else if (k <= 8) RBFFD_KNN(8);
     else if (k <= 16) RBFFD_KNN(16);
