@@ -1561,6 +1561,82 @@ int rbffd_spmv_device(rbffd_context *c, int64_t M, const int64_t *rowptr, const 
     });
 }
 
+struct rbffd_spmv_plan {
+    int device = 0;
+    rbffd::SpmvPlan p;
+};
+
+int rbffd_spmv_plan_create(rbffd_context *c, int64_t M, int64_t N, const int64_t *rowptr, const int32_t *col,
+                           const void *vals, int32_t vals_type, int32_t uniform_n, const int32_t *row_order,
+                           rbffd_spmv_plan **plan, int64_t *info)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!plan) throw Error{RBFFD_ERR_ARG, "spmv plan: null pointer"};
+        *plan = nullptr;
+        if (M < 0 || N < 0 || (M > 0 && (!col || !vals))) throw Error{RBFFD_ERR_ARG, "spmv plan: bad arguments"};
+        if (uniform_n <= 0 && !rowptr) throw Error{RBFFD_ERR_ARG, "spmv plan: rowptr or uniform_n needed"};
+        if (vals_type != RBFFD_F64 && vals_type != RBFFD_F32) throw Error{RBFFD_ERR_ARG, "spmv plan: value type"};
+        if (M >= (int64_t)INT_MAX) throw Error{RBFFD_ERR_ARG, "spmv plan: needs M < 2^31 - 1"};
+        auto q = std::make_unique<rbffd_spmv_plan>();
+        q->device = c->device;
+        rbffd::spmv_plan_build(q->p, M, N, rowptr, col, vals, vals_type == RBFFD_F32, uniform_n, row_order,
+                               c->stream);
+        if (info) {
+            info[0] = q->p.nblocks;
+            info[1] = q->p.hmax;
+            info[2] = q->p.overflow_blocks;
+            info[3] = q->p.nnz;
+        }
+        *plan = q.release();
+    });
+}
+
+int rbffd_spmv_plan_apply(rbffd_context *c, const rbffd_spmv_plan *plan, int32_t F, const double *x, double *y,
+                          double alpha, double beta)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!plan || (plan->p.M > 0 && (!x || !y))) throw Error{RBFFD_ERR_ARG, "spmv plan: null pointer"};
+        if (plan->device != c->device) throw Error{RBFFD_ERR_ARG, "spmv plan: made on another device"};
+        if (F != 1 && F != 2 && F != 4 && F != 8) throw Error{RBFFD_ERR_UNSUPPORTED, "spmv plan: fields must be 1, 2, 4 or 8"};
+        rbffd::launch_spmv_plan(plan->p, F, x, y, alpha, beta, c->stream);
+        RBFFD_HIP(hipGetLastError());
+    });
+}
+
+int rbffd_spmv_plan_destroy(rbffd_spmv_plan *plan)
+{
+    return guarded([&] {
+        if (!plan) return;
+        RBFFD_HIP(hipSetDevice(plan->device));
+        rbffd::spmv_plan_free(plan->p);
+        delete plan;
+    });
+}
+
+int rbffd_ssprk3_plan_device(rbffd_context *c, const rbffd_spmv_plan *plan, int32_t F, double *u, double *work,
+                             const uint8_t *fixed, double dt, int64_t steps)
+{
+    return guarded([&] {
+        check_ctx(c);
+        if (!plan || steps < 0 || (plan->p.M > 0 && (!u || !work))) throw Error{RBFFD_ERR_ARG, "ssprk3: bad arguments"};
+        if (plan->device != c->device) throw Error{RBFFD_ERR_ARG, "ssprk3: plan made on another device"};
+        if (plan->p.M != plan->p.N) throw Error{RBFFD_ERR_ARG, "ssprk3: the operator must be square"};
+        if (F != 1 && F != 2 && F != 4 && F != 8) throw Error{RBFFD_ERR_UNSUPPORTED, "ssprk3: fields must be 1, 2, 4 or 8"};
+        hipStream_t s = c->stream;
+        const rbffd::SpmvPlan &P = plan->p;
+        double *u1 = work, *u2 = work + (size_t)P.M * F;
+        for (int64_t k = 0; k < steps; ++k) {
+            // the stages of rbffd_ssprk3_device
+            rbffd::launch_rk_stage_plan(P, F, u, u, u1, fixed, 0.0, 1.0, dt, s);
+            rbffd::launch_rk_stage_plan(P, F, u1, u, u2, fixed, 0.75, 0.25, 0.25 * dt, s);
+            rbffd::launch_rk_stage_plan(P, F, u2, u, u, fixed, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, s);
+        }
+        RBFFD_HIP(hipGetLastError());
+    });
+}
+
 int rbffd_generate_operator(rbffd_context *c, const double *X, int64_t N, const double *Y, int64_t M, int32_t p,
                             int32_t n, int32_t polydeg, const rbffd_boundary *bnd, int32_t nops,
                             const int32_t *ops, const int32_t *opk, int32_t precision, int32_t index_base,

@@ -196,6 +196,38 @@ void launch_f64_to_f32(const double *src, long n, float *dst, hipStream_t s);
 // *out = sum of x[0 .. n) by a read-only stream over cus * 8 blocks (x 16-byte aligned)
 void launch_read_sum(const double *x, long n, double *out, int cus, hipStream_t s);
 
+// Application plan for repeated products with one CSR operator
+// (spmv_plan.hip): rows in blocks of kPlanRows consecutive rows of a locality
+// order, each block's distinct columns listed (at most kPlanHaloCap, sorted)
+// and its nonzeros given 16-bit local indices into that list.
+constexpr int kPlanRows = 256;
+constexpr int kPlanHaloCap = 2048;
+struct SpmvPlan {
+    long M = 0, N = 0, nnz = 0;
+    int nblocks = 0;
+    int hmax = 0;             // largest halo among the listed blocks
+    int overflow_blocks = 0;  // blocks with more than kPlanHaloCap distinct columns
+    int uniform = 0;          // entries of every row when all rows have as many, else 0
+    bool f32 = false;
+    int *prow = nullptr;      // plan row -> operator row
+    int64_t *pptr = nullptr;  // M + 1, the rows in plan order
+    uint16_t *plc = nullptr;  // nnz local column indices
+    int *pcol = nullptr;      // nnz columns; kept only when some block overflows
+    void *pval = nullptr;     // nnz values (FP64 or FP32), plan order
+    int *halo = nullptr;      // nblocks x kPlanHaloCap
+    int *hcount = nullptr;    // nblocks (+ 2 scratch)
+};
+// Allocates the plan's arrays (hipMalloc) and synchronises s.
+void spmv_plan_build(SpmvPlan &P, long M, long N, const int64_t *rowptr, const int *col, const void *vals,
+                     bool f32, int uniform_n, const int *order, hipStream_t s);
+void spmv_plan_free(SpmvPlan &P);
+// y = alpha A x + beta y; bitwise equal to launch_spmv with the default lanes.
+void launch_spmv_plan(const SpmvPlan &P, int F, const double *x, double *y, double alpha, double beta,
+                      hipStream_t s);
+// launch_rk_stage through a plan; bitwise equal to it.
+void launch_rk_stage_plan(const SpmvPlan &P, int F, const double *w, const double *u, double *out,
+                          const uint8_t *fixed, double a, double b, double c, hipStream_t s);
+
 // One SSP-RK stage out = a u + b w + c (L w) (rows with fixed != 0 keep u).
 void launch_rk_stage(long M, const int64_t *rowptr, const int *col, const void *vals, int vals_f32, int F,
                      int uniform_n, const int *order, const double *w, const double *u, double *out,

@@ -125,6 +125,10 @@ SIGNATURES = {
                                                ctypes.POINTER(_I64)]),
     "rbffd_spmv_device": (ctypes.c_int, [_V, _I64, _V, _V, _V, _I32, _I32, _V, _V, _D, _D, _I32, _V]),
     "rbffd_locality_order_device": (ctypes.c_int, [_V, _V, _I64, _V]),
+    "rbffd_spmv_plan_create": (ctypes.c_int, [_V, _I64, _I64, _V, _V, _V, _I32, _I32, _V, ctypes.POINTER(_V), _V]),
+    "rbffd_spmv_plan_apply": (ctypes.c_int, [_V, _V, _I32, _V, _V, _D, _D]),
+    "rbffd_spmv_plan_destroy": (ctypes.c_int, [_V]),
+    "rbffd_ssprk3_plan_device": (ctypes.c_int, [_V, _V, _I32, _V, _V, _V, _D, _I64]),
     "rbffd_read_sum_device": (ctypes.c_int, [_V, _V, _I64, _V]),
     "rbffd_lsq_solve_device": (ctypes.c_int, [_V, _I64, _I64, _V, _V, _V, _V, _V, _I32, _D, _I64,
                                               ctypes.POINTER(_I64), ctypes.POINTER(_D)]),

@@ -366,11 +366,69 @@ class Engine:
         check(self.L.rbffd_read_sum_device(self.ctx.handle, x.ptr, x.size, out.ptr))
         return out
 
+    def spmv_plan(self, col, vals, M, N, rowptr=None, uniform_n=0, order=None):
+        """An application plan of the CSR operator (M x N) for repeated
+        products (rbffd_spmv_plan_create); see SpmvPlan."""
+        return SpmvPlan(self, col, vals, M, N, rowptr=rowptr, uniform_n=uniform_n, order=order)
+
     def locality_order(self, X):
         """Morton order of the nodes of X (device, N x 2), as an int32 device array."""
         order = self.ctx.empty((X.shape[0],), np.int32)
         check(self.L.rbffd_locality_order_device(self.ctx.handle, X.ptr, X.shape[0], order.ptr))
         return order
+
+
+class SpmvPlan:
+    """An application plan of one CSR operator (rbffd_spmv_plan_create): the
+    rows in blocks of 256 of `order` (a locality order such as
+    Engine.locality_order of the rows' nodes), each block's distinct columns
+    listed, so a product stages a block's x rows in LDS.  Products equal
+    Engine.spmv bit for bit.  `info`: row blocks, largest listed halo, blocks
+    over 2048 distinct columns, nonzeros.  The plan keeps no reference to the
+    arrays it was made from."""
+
+    def __init__(self, eng, col, vals, M, N, rowptr=None, uniform_n=0, order=None):
+        self.eng, self.ctx, self.L = eng, eng.ctx, eng.L
+        self.shape = (int(M), int(N))
+        vt = 1 if vals.dtype == np.float32 else 0
+        h = ctypes.c_void_p()
+        info = (ctypes.c_int64 * 4)()
+        check(self.L.rbffd_spmv_plan_create(self.ctx.handle, int(M), int(N),
+                                            rowptr.ptr if rowptr is not None else None, col.ptr, vals.ptr, vt,
+                                            int(uniform_n), order.ptr if order is not None else None,
+                                            ctypes.byref(h), ctypes.cast(info, ctypes.c_void_p)))
+        self.handle = h
+        self.info = {"blocks": info[0], "max_halo": info[1], "overflow_blocks": info[2], "nnz": info[3]}
+
+    def apply(self, x, y, alpha=1.0, beta=0.0):
+        """y = alpha A x + beta y (device arrays, 1, 2, 4 or 8 fields node-major)."""
+        if x.shape[0] != self.shape[1] or y.shape[0] != self.shape[0] or x.shape[1:] != y.shape[1:]:
+            raise ValueError("SpmvPlan: shape mismatch")
+        F = 1 if len(x.shape) == 1 else x.shape[1]
+        check(self.L.rbffd_spmv_plan_apply(self.ctx.handle, self.handle, F, x.ptr, y.ptr, float(alpha),
+                                           float(beta)))
+        return y
+
+    def ssprk3(self, u, dt, steps, fixed=None, work=None):
+        """Engine.ssprk3 with this plan's operator and row order."""
+        N = u.shape[0]
+        F = 1 if len(u.shape) == 1 else u.shape[1]
+        if work is None:
+            work = self.ctx.empty((2 * N * F,), np.float64)
+        check(self.L.rbffd_ssprk3_plan_device(self.ctx.handle, self.handle, F, u.ptr, work.ptr,
+                                              fixed.ptr if fixed is not None else None, float(dt), int(steps)))
+        return u
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            check(self.L.rbffd_spmv_plan_destroy(self.handle))
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceOperator:
