@@ -368,9 +368,8 @@ void knn_enqueue(rbffd_arena &A, bool may_alloc, const double *X, long N, const 
     int *redo2 = (int *)grow(A.b[A_REDO2], sizeof(int) * (M + 1), may_alloc);
     rw.count2 = redo2;
     rw.list2 = redo2 + 1;
-    rbffd::launch_knn_query(grid, start, cxy, cidx, cgrp, N, Q, qgroup, qorder, M, k, idx, d2, redo + 1, redo, rw,
-                            s);
-    count_negative_kernel<<<nblk(M * (long)k), 256, 0, s>>>(idx, M * (long)k, flag + 1);
+    rbffd::launch_knn_query(grid, start, cxy, cidx, cgrp, N, Q, qgroup, qorder, M, k, idx, d2, flag + 1, redo + 1,
+                            redo, rw, s);
     RBFFD_HIP(hipGetLastError());
 }
 

@@ -53,7 +53,8 @@ void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tm
 void launch_cell_scatter(const double *X, const int *xgroup, long N, const int *cell_of, const int *start,
                          int *fill, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s);
 // Exact kNN of every query, ascending (d^2, index), over the N points of the
-// cell arrays.  redo (M ints, may be
+// cell arrays; *unfilled (may be NULL) gains the result slots left at -1 (a
+// query that sees fewer than k nodes, or non-finite points).  redo (M ints, may be
 // NULL) and redo_count (one int) are the workspace of the two-phase selection
 // used for k > 8; without them every query runs the insertion kernel.  With
 // them, rw holds the queries' k-th distances (M doubles) and the list (M ints)
@@ -65,7 +66,7 @@ struct KnnRedo {
 };
 void launch_knn_query(const GridParams *g, const int *cell_start, const double *cell_xy, const int *cell_idx,
                       const int *cell_grp, long N, const double *Q, const int *qgroup, const int *qorder,
-                      long M, int k, int *idx_out, double *d2_out, int *redo, int *redo_count,
+                      long M, int k, int *idx_out, double *d2_out, int *unfilled, int *redo, int *redo_count,
                       const KnnRedo &rw, hipStream_t s);
 
 // ------------------------------------------------------- local solves (K2)

@@ -210,7 +210,8 @@ __global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__rest
                                                         const int *__restrict__ qgroup,
                                                         const int *__restrict__ qorder, long M,
                                                         const int *__restrict__ mcount, int k,
-                                                        int *__restrict__ idx_out, double *__restrict__ d2_out)
+                                                        int *__restrict__ idx_out, double *__restrict__ d2_out,
+                                                        int *__restrict__ unfilled)
 {
 #pragma clang fp contract(off)
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -222,6 +223,7 @@ __global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__rest
             idx_out[q * k + j] = -1;
             if (d2_out) d2_out[q * k + j] = INFINITY;
         }
+        if (unfilled) atomicAdd(unfilled, k);
         return;
     }
     const double qx = Q[2 * q], qy = Q[2 * q + 1];
@@ -300,13 +302,16 @@ __global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__rest
         if (qcx - r <= 0 && qcy - r <= 0 && qcx + r >= g.gx - 1 && qcy + r >= g.gy - 1) break;
     }
 
+    int missing = 0;
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
         if (j >= KMAX - k) {
             idx_out[q * k + (j - (KMAX - k))] = ki[j];
             if (d2_out) d2_out[q * k + (j - (KMAX - k))] = kd[j];
+            missing += ki[j] < 0 ? 1 : 0;
         }
     }
+    if (unfilled && missing) atomicAdd(unfilled, missing);
 }
 
 // kv = min(kv, c), c = max(kv, c): two instructions.  __builtin_fmin/fmax
@@ -672,7 +677,7 @@ __global__ __launch_bounds__(64 * kTieWaves) void knn_tie_kernel(
 template <int KMAX>
 static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
                          long N, const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
-                         double *d2, int *redo, int *redo_count, const KnnRedo &rw, hipStream_t s)
+                         double *d2, int *unfilled, int *redo, int *redo_count, const KnnRedo &rw, hipStream_t s)
 {
     const unsigned blocks = (unsigned)((M + 255) / 256);
     if (KMAX >= 16 && redo) {
@@ -687,16 +692,16 @@ static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, 
         knn_tie_kernel<<<tblocks, 64 * kTieWaves, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, redo, rw.tau,
                                                            redo_count, k, idx, d2, rw.list2, rw.count2);
         knn_query_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, rw.list2, M, rw.count2, k,
-                                                       idx, d2);
+                                                       idx, d2, unfilled);
     } else {
         knn_query_kernel<KMAX><<<blocks, 256, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, nullptr, k,
-                                                       idx, d2);
+                                                       idx, d2, unfilled);
     }
 }
 
 void launch_knn_query(const GridParams *g, const int *cs, const double *cxy, const int *cidx, const int *cgrp,
                       long N, const double *Q, const int *qgroup, const int *qorder, long M, int k, int *idx,
-                      double *d2, int *redo, int *redo_count, const KnnRedo &rw, hipStream_t s)
+                      double *d2, int *unfilled, int *redo, int *redo_count, const KnnRedo &rw, hipStream_t s)
 {
     if (M <= 0) return;
     if (redo) {
@@ -704,7 +709,7 @@ void launch_knn_query(const GridParams *g, const int *cs, const double *cxy, con
         launch_fill_i32(rw.count2, 1, 0, s);
     }
 #define RBFFD_KNN(KM) \
-    knn_launch_t<KM>(g, cs, cxy, cidx, cgrp, N, Q, qgroup, qorder, M, k, idx, d2, redo, redo_count, rw, s)
+    knn_launch_t<KM>(g, cs, cxy, cidx, cgrp, N, Q, qgroup, qorder, M, k, idx, d2, unfilled, redo, redo_count, rw, s)
     if (k <= 1) RBFFD_KNN(1);
     else if (k <= 8) RBFFD_KNN(8);
     else if (k <= 16) RBFFD_KNN(16);

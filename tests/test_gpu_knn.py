@@ -87,3 +87,23 @@ def test_knn_full_size_1m(ctx, O):
     ri, rd = O.knn(X, X, 31, method="grid", nthreads=16)
     assert np.array_equal(idx, ri)
     assert np.array_equal(d2, rd)
+
+
+@pytest.mark.parametrize("k", [5, 8, 20])
+def test_knn_masked_query_with_too_few_visible_nodes(ctx, O, k):
+    # the 6 nodes of group 2 see only each other (no group-0 nodes), so with
+    # k > 6 their searches come up short and the call is an error; with k <= 6
+    # every query is filled and equals brute force
+    rng = np.random.default_rng(9)
+    X = rng.uniform(0, 1, size=(300, 2))
+    g = np.ones(300, np.int32)
+    g[:6] = 2
+    print(f"knn short masked queries k={k}", flush=True)
+    if k > 6:
+        with pytest.raises(ValueError):
+            _run(ctx, X, X, k, g, g)
+    else:
+        idx, d2 = _run(ctx, X, X, k, g, g)
+        ri, rd = O.knn(X, X, k, g, g, method="brute")
+        assert np.array_equal(idx, ri)
+        assert np.array_equal(d2, rd)
