@@ -105,3 +105,36 @@ def test_batch_reports_the_failing_set(ctx):
     # the context still works
     out, st, _ = eng.generate_batch([sets[0], sets[2]], OPS, use_graph=True)
     assert not st.any()
+
+
+def test_batch_graph_after_an_eager_call_with_bigger_sets(ctx):
+    # graph A, an eager batch B whose sets are larger (its workspace grows and
+    # the buffers A's graph recorded are reallocated), then graph A again:
+    # the second A must not replay the stale graph; its outputs equal A's first
+    import rbffd_amd
+
+    eng = rbffd_amd.Engine(ctx)
+    small = [(1200, 2, False), (900, 3, True), (1500, 4, False)]
+    big = [(9000, 4, False), (7000, 5, True), (12000, 2, False)]
+    sets_a, _ = _sets(ctx, rbffd_amd, small)
+    sets_b, _ = _sets(ctx, rbffd_amd, big)
+    print("graph A", flush=True)
+    out, st, _ = eng.generate_batch(sets_a, OPS, use_graph=True)
+    assert not st.any()
+    first = [(c.get(), [v.get() for v in vs]) for c, vs, _ in out]
+    for S, (c, vs, _) in zip(sets_a, out):
+        S["colidx"], S["vals"] = c, vs
+    print("eager B (bigger sets)", flush=True)
+    _, stb, _ = eng.generate_batch(sets_b, OPS, use_graph=False)
+    assert not stb.any()
+    for c, vs, _ in out:
+        c.zero_()
+        for v in vs:
+            v.zero_()
+    print("graph A again", flush=True)
+    out2, st2, _ = eng.generate_batch(sets_a, OPS, use_graph=True)
+    assert not st2.any()
+    for (c, vs, _), (cf, vsf) in zip(out2, first):
+        assert np.array_equal(c.get(), cf)
+        for v, vf in zip(vs, vsf):
+            assert np.array_equal(v.get(), vf)
