@@ -111,3 +111,38 @@ def test_host_entry_point_one_based(ctx, O):
         err = (num / den).max()
         print(f"  {name}: nzval vs quad {err:.2e}", flush=True)
         assert err <= 1e-10
+
+
+def test_host_entry_point_large_rows_widened_on_the_host(ctx):
+    # 600,625 rows x 31 entries: past two staging buffers of row indices, so
+    # the int32 rows cross in chunks and the host widens them (index base 1);
+    # the result equals the device-side CSC of the same rows
+    import ctypes
+
+    import rbffd_amd
+    from rbffd_amd import _lib
+
+    L = _lib.load()
+    X = jittered_grid(775, 775, 0.25, seed=12)
+    N, n = len(X), 31
+    print(f"host entry point at {N} rows", flush=True)
+    ops = np.array([_lib.OP["Lap"]], np.int32)
+    ks = np.zeros(1, np.int32)
+    colptr = np.empty(N + 1, np.int64)
+    rowval = np.empty(N * n, np.int64)
+    nz = np.empty(N * n)
+    nzp = (ctypes.c_void_p * 1)(nz.ctypes.data)
+    ncols = ctypes.c_int64()
+    code = L.rbffd_generate_operator(ctx.handle, X.ctypes.data, N, X.ctypes.data, N, 7, n, 4, None, 1,
+                                     ops.ctypes.data, ks.ctypes.data, 0, 1, colptr.ctypes.data,
+                                     rowval.ctypes.data, ctypes.cast(nzp, ctypes.c_void_p), ctypes.byref(ncols),
+                                     None)
+    _lib.check(code)
+    eng = rbffd_amd.Engine(ctx)
+    Xd = ctx.to_device(X)
+    col, vals, _ = eng.generate(Xd, Xd, 7, n, 4, ["Lap"])
+    cp, rv, cv, used = eng.csr_to_csc(col, vals, N, index_base=1)
+    assert ncols.value == used
+    assert np.array_equal(colptr, cp.get())
+    assert np.array_equal(rowval, rv.get())
+    assert np.array_equal(nz, cv[0].get())
