@@ -18,6 +18,8 @@
 #include <tuple>
 #include <vector>
 
+#include <sched.h>
+
 #include "internal.hpp"
 #include "rbffd.h"
 
@@ -292,6 +294,21 @@ int read_flag(rbffd_context *c, const int *dflag)
 // previous chunk into place.  Synchronous.
 constexpr size_t kStageBytes = 32u << 20;
 
+// Host threads that copy staged chunks into the caller's arrays: the job's CPU
+// share (RBFFD_HOST_THREADS, else OMP_NUM_THREADS, else the CPUs this process
+// may run on), at most 32.  Writes into pageable memory bound the copy-out:
+// 1.05 GB took 45 ms with 8 threads and 37 ms with 16 on a 16-CPU share.
+int host_copy_threads()
+{
+    for (const char *name : {"RBFFD_HOST_THREADS", "OMP_NUM_THREADS"}) {
+        const char *e = std::getenv(name);
+        if (e && std::atoi(e) > 0) return std::min(32, std::atoi(e));
+    }
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return std::max(1, std::min(32, CPU_COUNT(&set)));
+    return (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+}
+
 void d2h_staged(rbffd_context *c, void *dst, const void *src, size_t bytes)
 {
     hipStream_t s = c->stream;
@@ -304,10 +321,7 @@ void d2h_staged(rbffd_context *c, void *dst, const void *src, size_t bytes)
         if (!c->stage[b]) RBFFD_HIP(hipHostMalloc(&c->stage[b], kStageBytes, hipHostMallocDefault));
         if (!c->stage_ev[b]) RBFFD_HIP(hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming));
     }
-    if (!c->pool) {
-        const unsigned hw = std::thread::hardware_concurrency();
-        c->pool = std::make_unique<rbffd_host_pool>((int)std::max(1u, std::min(8u, hw)));
-    }
+    if (!c->pool) c->pool = std::make_unique<rbffd_host_pool>(host_copy_threads());
     const size_t nchunk = (bytes + kStageBytes - 1) / kStageBytes;
     auto enqueue = [&](size_t i) {
         const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
