@@ -219,66 +219,105 @@ __global__ __launch_bounds__(NT) void plan_apply_kernel(long M, int nu, const in
     __syncthreads();
     const int sub = tid & (G - 1), grp = tid / G;
     const int fq = sub % FL, nzp = sub / FL;
-    // every lane runs every pass: the reductions below need the whole 16-lane group
-    for (int i0 = 0; i0 < nr; i0 += NT / G) {
-        const int i = i0 + grp;
-        const bool live = i < nr;
-        long p0 = 0, p1 = 0, row = 0;
-        if (live) {
-            p0 = nu > 0 ? (r0 + i) * (long)nu : pptr[r0 + i];
-            p1 = nu > 0 ? p0 + nu : pptr[r0 + i + 1];
-            row = prow[r0 + i];
-        }
-        double acc[FPL];
+    // Each group takes two rows a pass (i and i + NT/G), so a lane has twice the
+    // loads in flight; with rows of nu entries both rows' nonzeros run in one
+    // loop.  Every lane runs every pass: the reductions need the whole group.
+    constexpr int STRIDE = NT / G;
+    for (int i0 = 0; i0 < nr; i0 += 2 * STRIDE) {
+        long p0[2], p1[2], row[2];
+        bool live[2];
 #pragma unroll
-        for (int f = 0; f < FPL; ++f) acc[f] = 0.0;
+        for (int h = 0; h < 2; ++h) {
+            const int i = i0 + grp + h * STRIDE;
+            live[h] = i < nr;
+            p0[h] = p1[h] = row[h] = 0;
+            if (live[h]) {
+                p0[h] = nu > 0 ? (r0 + i) * (long)nu : pptr[r0 + i];
+                p1[h] = nu > 0 ? p0[h] + nu : pptr[r0 + i + 1];
+                row[h] = prow[r0 + i];
+            }
+        }
+        double acc[2][FPL];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int f = 0; f < FPL; ++f) acc[h][f] = 0.0;
         // the same nonzeros in the same order as the plain kernels
-        if (staged) {
-            for (long p = p0 + nzp; p < p1; p += NZP) {
-                const double v = (double)pval[p];
-                const double *xr = xs + (int)plc[p] * SX + fq * FPL;
+        if (staged && nu > 0) {
+            for (int j = nzp; j < nu; j += NZP) {
+                double v[2];
+                int c[2];
 #pragma unroll
-                for (int f = 0; f < FPL; ++f) acc[f] = __builtin_fma(v, xr[f], acc[f]);
+                for (int h = 0; h < 2; ++h) {
+                    v[h] = live[h] ? (double)pval[p0[h] + j] : 0.0;
+                    c[h] = live[h] ? (int)plc[p0[h] + j] : 0;
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!live[h]) continue;
+                    const double *xr = xs + c[h] * SX + fq * FPL;
+#pragma unroll
+                    for (int f = 0; f < FPL; ++f) acc[h][f] = __builtin_fma(v[h], xr[f], acc[h][f]);
+                }
             }
         } else {
-            for (long p = p0 + nzp; p < p1; p += NZP) {
-                const double v = (double)pval[p];
-                const long c = listed ? (long)hb[plc[p]] : (long)pcol[p];
-                const double *xr = x + c * F + fq * FPL;
 #pragma unroll
-                for (int f = 0; f < FPL; ++f) acc[f] = __builtin_fma(v, xr[f], acc[f]);
+            for (int h = 0; h < 2; ++h) {
+                if (staged) {
+                    for (long p = p0[h] + nzp; p < p1[h]; p += NZP) {
+                        const double v = (double)pval[p];
+                        const double *xr = xs + (int)plc[p] * SX + fq * FPL;
+#pragma unroll
+                        for (int f = 0; f < FPL; ++f) acc[h][f] = __builtin_fma(v, xr[f], acc[h][f]);
+                    }
+                } else {
+                    for (long p = p0[h] + nzp; p < p1[h]; p += NZP) {
+                        const double v = (double)pval[p];
+                        const long cc = listed ? (long)hb[plc[p]] : (long)pcol[p];
+                        const double *xr = x + cc * F + fq * FPL;
+#pragma unroll
+                        for (int f = 0; f < FPL; ++f) acc[h][f] = __builtin_fma(v, xr[f], acc[h][f]);
+                    }
+                }
             }
         }
-        if constexpr (FL == 1) {
 #pragma unroll
-            for (int f = 0; f < F; ++f) acc[f] = group_sum_f64<G>(acc[f]);
-        } else {
+        for (int h = 0; h < 2; ++h) {
+            if constexpr (FL == 1) {
 #pragma unroll
-            for (int f = 0; f < FPL; ++f) {
-                if constexpr (NZP >= 8) acc[f] += dpp_f64<0x120 | (FL)>(acc[f]);  // row_ror
-                if constexpr (NZP >= 4) acc[f] += dpp_f64<0x120 | (G / 4)>(acc[f]);
-                if constexpr (NZP >= 2) acc[f] += dpp_f64<0x120 | (G / 2)>(acc[f]);
-            }
-        }
-        if (!live) continue;
-        auto put = [&](int f, double s) {
-            const long e = row * F + f;
-            if constexpr (MODE == 0) {
-                double r = o.alpha * s;
-                if (o.beta != 0.0) r = __builtin_fma(o.beta, o.y[e], r);
-                o.y[e] = r;
+                for (int f = 0; f < F; ++f) acc[h][f] = group_sum_f64<G>(acc[h][f]);
             } else {
-                const double ui = o.u[e];
-                o.y[e] = (o.fixed && o.fixed[row]) ? ui : o.a * ui + o.b * x[e] + o.c * s;
+#pragma unroll
+                for (int f = 0; f < FPL; ++f) {
+                    if constexpr (NZP >= 8) acc[h][f] += dpp_f64<0x120 | (FL)>(acc[h][f]);  // row_ror
+                    if constexpr (NZP >= 4) acc[h][f] += dpp_f64<0x120 | (G / 4)>(acc[h][f]);
+                    if constexpr (NZP >= 2) acc[h][f] += dpp_f64<0x120 | (G / 2)>(acc[h][f]);
+                }
             }
-        };
-        if constexpr (FL == 1) {
+        }
 #pragma unroll
-            for (int f = 0; f < F; ++f)
-                if (f % G == sub) put(f, acc[f]);
-        } else if (nzp == 0) {
+        for (int h = 0; h < 2; ++h) {
+            if (!live[h]) continue;
+            const long rw = row[h];
+            auto put = [&](int f, double sum) {
+                const long e = rw * F + f;
+                if constexpr (MODE == 0) {
+                    double r = o.alpha * sum;
+                    if (o.beta != 0.0) r = __builtin_fma(o.beta, o.y[e], r);
+                    o.y[e] = r;
+                } else {
+                    const double ui = o.u[e];
+                    o.y[e] = (o.fixed && o.fixed[rw]) ? ui : o.a * ui + o.b * x[e] + o.c * sum;
+                }
+            };
+            if constexpr (FL == 1) {
 #pragma unroll
-            for (int f = 0; f < FPL; ++f) put(fq * FPL + f, acc[f]);
+                for (int f = 0; f < F; ++f)
+                    if (f % G == sub) put(f, acc[h][f]);
+            } else if (nzp == 0) {
+#pragma unroll
+                for (int f = 0; f < FPL; ++f) put(fq * FPL + f, acc[h][f]);
+            }
         }
     }
 }
