@@ -180,7 +180,8 @@ template <int F>
 constexpr int plan_lds_stride() { return F == 1 ? 1 : F + 2; }
 
 // 16 lanes a row, FL lanes a nonzero (FL = 1: every field in every lane), NT
-// lanes a block, so a block's kPlanRows rows run in kPlanRows * 16 / NT passes.
+// lanes a block, so a block's kPlanRows rows run in kPlanRows * 16 / (2 NT)
+// passes.
 // Rows of nu entries each (nu > 0) need no row pointers.  xs: the block's halo
 // rows of x in LDS when the halo fits (staged).
 template <typename TV, int F, int FL, int MODE, int NT>
@@ -330,9 +331,9 @@ void plan_apply_t(const SpmvPlan &P, const double *x, const PlanOut &o, hipStrea
     const int cap = std::min(P.hmax, (int)(65536 / (sizeof(double) * SX)));
     const size_t lds = sizeof(double) * SX * (size_t)std::max(cap, 1);
     const bool x16 = ((uintptr_t)x & 15) == 0;
-    // 1024 lanes a block from 4 fields up (four passes of 64 rows: at 8 fields
-    // 161 us per product over 1M rows against 318 us with 256 lanes), 256 below
-    // (one field: 84-95 us either way)
+    // 1024 lanes a block from 4 fields up (two passes of 128 rows: at 8 fields
+    // 161-168 us per product over 1M rows, against 318 us with 256 lanes a
+    // block), 256 below (one field: 85-95 us either way)
     constexpr int NT = F >= 4 ? 1024 : 256;
     plan_apply_kernel<TV, F, FL, MODE, NT><<<P.nblocks, NT, lds, s>>>(
         P.M, P.uniform, P.prow, P.pptr, P.plc, P.pcol, (const TV *)P.pval, P.halo, P.hcount, cap, x16, x, o);
