@@ -180,11 +180,11 @@ template <int F>
 constexpr int plan_lds_stride() { return F == 1 ? 1 : F + 2; }
 
 // 16 lanes a row, FL lanes a nonzero (FL = 1: every field in every lane), NT
-// lanes a block, so a block's kPlanRows rows run in kPlanRows * 16 / (2 NT)
-// passes.
+// lanes a block and RPP rows per lane group, so a block's kPlanRows rows run
+// in kPlanRows * 16 / (RPP NT) passes.
 // Rows of nu entries each (nu > 0) need no row pointers.  xs: the block's halo
 // rows of x in LDS when the halo fits (staged).
-template <typename TV, int F, int FL, int MODE, int NT>
+template <typename TV, int F, int FL, int MODE, int NT, int RPP>
 __global__ __launch_bounds__(NT) void plan_apply_kernel(long M, int nu, const int *__restrict__ prow,
                                                         const int64_t *__restrict__ pptr,
                                                         const uint16_t *__restrict__ plc,
@@ -220,15 +220,16 @@ __global__ __launch_bounds__(NT) void plan_apply_kernel(long M, int nu, const in
     __syncthreads();
     const int sub = tid & (G - 1), grp = tid / G;
     const int fq = sub % FL, nzp = sub / FL;
-    // Each group takes two rows a pass (i and i + NT/G), so a lane has twice the
-    // loads in flight; with rows of nu entries both rows' nonzeros run in one
-    // loop.  Every lane runs every pass: the reductions need the whole group.
+    // Each group takes RPP rows a pass (i, i + NT/G, ...), so a lane has RPP
+    // times the loads in flight; with rows of nu entries the rows' nonzeros run
+    // in one loop.  Every lane runs every pass: the reductions need the whole
+    // group.
     constexpr int STRIDE = NT / G;
-    for (int i0 = 0; i0 < nr; i0 += 2 * STRIDE) {
-        long p0[2], p1[2], row[2];
-        bool live[2];
+    for (int i0 = 0; i0 < nr; i0 += RPP * STRIDE) {
+        long p0[RPP], p1[RPP], row[RPP];
+        bool live[RPP];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < RPP; ++h) {
             const int i = i0 + grp + h * STRIDE;
             live[h] = i < nr;
             p0[h] = p1[h] = row[h] = 0;
@@ -238,23 +239,23 @@ __global__ __launch_bounds__(NT) void plan_apply_kernel(long M, int nu, const in
                 row[h] = prow[r0 + i];
             }
         }
-        double acc[2][FPL];
+        double acc[RPP][FPL];
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < RPP; ++h)
 #pragma unroll
             for (int f = 0; f < FPL; ++f) acc[h][f] = 0.0;
         // the same nonzeros in the same order as the plain kernels
         if (staged && nu > 0) {
             for (int j = nzp; j < nu; j += NZP) {
-                double v[2];
-                int c[2];
+                double v[RPP];
+                int c[RPP];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < RPP; ++h) {
                     v[h] = live[h] ? (double)pval[p0[h] + j] : 0.0;
                     c[h] = live[h] ? (int)plc[p0[h] + j] : 0;
                 }
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < RPP; ++h) {
                     if (!live[h]) continue;
                     const double *xr = xs + c[h] * SX + fq * FPL;
 #pragma unroll
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(NT) void plan_apply_kernel(long M, int nu, const in
             }
         } else {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < RPP; ++h) {
                 if (staged) {
                     for (long p = p0[h] + nzp; p < p1[h]; p += NZP) {
                         const double v = (double)pval[p];
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(NT) void plan_apply_kernel(long M, int nu, const in
             }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < RPP; ++h) {
             if constexpr (FL == 1) {
 #pragma unroll
                 for (int f = 0; f < F; ++f) acc[h][f] = group_sum_f64<G>(acc[h][f]);
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(NT) void plan_apply_kernel(long M, int nu, const in
             }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < RPP; ++h) {
             if (!live[h]) continue;
             const long rw = row[h];
             auto put = [&](int f, double sum) {
@@ -331,11 +332,14 @@ void plan_apply_t(const SpmvPlan &P, const double *x, const PlanOut &o, hipStrea
     const int cap = std::min(P.hmax, (int)(65536 / (sizeof(double) * SX)));
     const size_t lds = sizeof(double) * SX * (size_t)std::max(cap, 1);
     const bool x16 = ((uintptr_t)x & 15) == 0;
-    // 1024 lanes a block from 4 fields up (two passes of 128 rows: at 8 fields
-    // 161-168 us per product over 1M rows, against 318 us with 256 lanes a
-    // block), 256 below (one field: 85-95 us either way)
+    // 1024 lanes a block from 4 fields up (at 8 fields 161-168 us per product
+    // over 1M rows, against 318 us with 256 lanes a block), 256 below
     constexpr int NT = F >= 4 ? 1024 : 256;
-    plan_apply_kernel<TV, F, FL, MODE, NT><<<P.nblocks, NT, lds, s>>>(
+    // rows a lane group takes per pass: two up to 2 fields (one field: 85-86 us
+    // against 95), one from 4 fields up (two rows: 8 fields no faster, and an
+    // FP32 SSP-RK3 run 17% slower)
+    constexpr int RPP = F >= 4 ? 1 : 2;
+    plan_apply_kernel<TV, F, FL, MODE, NT, RPP><<<P.nblocks, NT, lds, s>>>(
         P.M, P.uniform, P.prow, P.pptr, P.plc, P.pcol, (const TV *)P.pval, P.halo, P.hcount, cap, x16, x, o);
 }
 

@@ -95,3 +95,7 @@ def test_device_operator_product(ctx, fitted_nodes, F):
         op = rbffd_amd.DeviceOperator(A, ctx=ctx)
         got2 = op @ x
         assert np.array_equal(got2.reshape(ref.shape), got.reshape(ref.shape))
+        # the same operator through an application plan, rows in a random order
+        order = np.random.default_rng(7).permutation(A.shape[0])
+        opp = rbffd_amd.DeviceOperator(A, ctx=ctx, plan=True, order=order)
+        assert np.array_equal((opp @ x).reshape(ref.shape), got.reshape(ref.shape))
