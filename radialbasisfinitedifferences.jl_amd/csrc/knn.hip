@@ -314,6 +314,11 @@ __global__ __launch_bounds__(256) void knn_query_kernel(const GridParams *__rest
     if (unfilled && missing) atomicAdd(unfilled, missing);
 }
 
+// Bits of a staged entry that hold the cell position when its rank (below
+// KMAX) shares the entry: 26 up to KMAX = 32, 25 up to 64.
+template <int KMAX>
+constexpr int knn_pos_bits() { return KMAX <= 32 ? 26 : 25; }
+
 // kv = min(kv, c), c = max(kv, c): two instructions.  __builtin_fmin/fmax
 // first re-quiet both operands for the IEEE mode kernels run in, a third
 // instruction a slot; no NaN reaches this chain (a NaN distance fails the
@@ -342,14 +347,14 @@ __device__ __forceinline__ void minmax_slot(double &kv, double &c)
 // d^2 the same way when it is asked for): stores made straight from phase 2
 // put 4 bytes into a different row per lane, spread over the whole cell walk,
 // and the partly written lines cost about 2.7x the output size in HBM writes.
-// Up to KMAX = 32 (and below 2^26 points) the rows are staged in the order
+// Below 2^26 points (2^25 from KMAX = 48) the rows are staged in the order
 // found and ranked after the walk, when every lane has its k candidates:
 // during the walk about a third of the lanes accept a given candidate, and
-// the rank costs the wavefront the same either way.  The rank is kept in bits
-// 26..30 of the staged entry, since storing each entry at its rank straight to
-// HBM from the owning lane is as scattered as storing during the walk.  Rows
-// sent to `redo` are not written here; the redo kernels write all k entries of
-// those.
+// the rank costs the wavefront the same either way.  The rank is kept above
+// the position bits of the staged entry, since storing each entry at its rank
+// straight to HBM from the owning lane is as scattered as storing during the
+// walk.  Rows sent to `redo` are not written here; the redo kernels write all
+// k entries of those.
 template <int KMAX, bool STAGE>
 __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__restrict__ gp,
                                                          const int *__restrict__ cell_start,
@@ -369,9 +374,10 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
     const long t0 = blockIdx.x * (long)blockDim.x;
     const long t = t0 + threadIdx.x;
     int *const mine = STAGE ? srow + threadIdx.x * k : nullptr;
-    // rank after the walk: k <= 32 and cell positions below 2^26 (pack), so a
-    // staged entry holds its position and its rank
-    constexpr bool DENSE_OK = STAGE && KMAX <= 32;
+    // rank after the walk when a staged entry can hold a cell position (PB
+    // bits: cell positions below 2^PB, which pack states) and its rank above it
+    constexpr bool DENSE_OK = STAGE;
+    constexpr int PB = knn_pos_bits<KMAX>();
     const bool dense = DENSE_OK && pack;
     // the query's row when every entry of it is placed, else -1
     const long qdone = [&]() -> long {
@@ -511,7 +517,7 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
             if (dense) {
                 // the k candidates, staged in the order found, ranked with
                 // every lane of the wavefront at work (four at a time, so their
-                // coordinate loads overlap); the rank goes to bits 26..30
+                // coordinate loads overlap); the rank goes above bit PB
                 int j = 0;
                 for (; j + 4 <= k; j += 4) {
                     int p[4];
@@ -521,9 +527,9 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
 #pragma unroll
                     for (int u = 0; u < 4; ++u) d[u] = d2of(p[u]);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) mine[j + u] = p[u] | rank(d[u]) << 26;
+                    for (int u = 0; u < 4; ++u) mine[j + u] = p[u] | rank(d[u]) << PB;
                 }
-                for (; j < k; ++j) mine[j] = mine[j] | rank(d2of(mine[j])) << 26;
+                for (; j < k; ++j) mine[j] = mine[j] | rank(d2of(mine[j])) << PB;
             }
         }
         return q;
@@ -545,8 +551,8 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
         if (qq >= 0) {
             int p = wrow[e], at = pos;
             if (dense) {
-                at = p >> 26;
-                p &= (1 << 26) - 1;
+                at = p >> PB;
+                p &= (1 << PB) - 1;
             }
             idx_out[(long)qq * k + at] = cell_idx[p];
             if (d2_out) {
@@ -686,7 +692,7 @@ static void knn_launch_t(const GridParams *g, const int *cs, const double *cxy, 
         const size_t lds = stage ? sizeof(int) * 256 * (size_t)k : 0;
         knn_select_kernel<KMAX, stage><<<blocks, 256, lds, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, qorder, M, k,
                                                                  idx, d2, redo, rw.tau, redo_count,
-                                                                 N <= (1L << 26) ? 1 : 0);
+                                                                 N <= (1L << knn_pos_bits<KMAX>()) ? 1 : 0);
         // at most 16384 wavefronts walk the tied queries in turn
         const unsigned tblocks = (unsigned)std::max(1L, std::min<long>((M + kTieWaves - 1) / kTieWaves, 4096));
         knn_tie_kernel<<<tblocks, 64 * kTieWaves, 0, s>>>(g, cs, cxy, cidx, cgrp, Q, qgroup, redo, rw.tau,
