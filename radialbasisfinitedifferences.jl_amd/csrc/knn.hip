@@ -403,26 +403,32 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
         const int qcy = (int)fmin(fmax(floor(fy), -2.0), (double)g.gy + 1.0);
         const int rlimit = g.gx + g.gy + 4;
 
-        // the cells of ring r in a fixed order; f(cell, lower bound on d^2)
+        // The cells of ring r as runs along a grid row: the cells of one row
+        // are contiguous in cell order, so a run is one range of points, and a
+        // wavefront walks the 3x3 core as three runs of three cells where it
+        // would otherwise set up nine cells, most holding fewer points than the
+        // fullest cell of the same step on another lane.  Step 0 is rings 0
+        // and 1 together; step r >= 2 is ring r: its top and bottom rows, then
+        // the side cells.  f(first cell, one past the last cell, lower bound
+        // on d^2 over the run).
         auto ring = [&](int r, auto &&f) {
-            const int ncell = r == 0 ? 1 : 8 * r;
-            for (int tc = 0; tc < ncell; ++tc) {
-                int dxc = 0, dyc = 0;
-                if (r > 0) {
-                    int side = tc / (2 * r), off = tc - side * 2 * r;
-                    if (side == 0) { dxc = -r + off; dyc = -r; }
-                    else if (side == 1) { dxc = r; dyc = -r + off; }
-                    else if (side == 2) { dxc = r - off; dyc = r; }
-                    else { dxc = -r; dyc = r - off; }
-                }
-                const int cx = qcx + dxc, cy = qcy + dyc;
-                if (cx < 0 || cx >= g.gx || cy < 0 || cy >= g.gy) continue;
-                const double rx0 = g.x0 + cx * g.csx, rx1 = rx0 + g.csx;
+            const int nrun = r == 0 ? 3 : 4 * r;
+            for (int s = 0; s < nrun; ++s) {
+                int a, b, dyc;
+                if (r == 0) { a = -1; b = 1; dyc = s - 1; }
+                else if (s < 2) { a = -r; b = r; dyc = s == 0 ? -r : r; }
+                else { a = b = ((s - 2) & 1) ? r : -r; dyc = -r + 1 + ((s - 2) >> 1); }
+                int cx0 = qcx + a, cx1 = qcx + b;
+                const int cy = qcy + dyc;
+                if (cx1 < 0 || cx0 >= g.gx || cy < 0 || cy >= g.gy) continue;
+                cx0 = max(cx0, 0);
+                cx1 = min(cx1, g.gx - 1);
+                const double rx0 = g.x0 + cx0 * g.csx, rx1 = g.x0 + (cx1 + 1) * g.csx;
                 const double ry0 = g.y0 + cy * g.csy, ry1 = ry0 + g.csy;
-                // the gap to the cell per axis (0 inside), without branches
+                // the gap to the run per axis (0 inside), without branches
                 const double ddx = fmax(fmax(rx0 - qx, qx - rx1), 0.0);
                 const double ddy = fmax(fmax(ry0 - qy, qy - ry1), 0.0);
-                f(cy * g.gx + cx, (ddx * ddx + ddy * ddy) * (1.0 - 1e-9));
+                f(cy * g.gx + cx0, cy * g.gx + cx1 + 1, (ddx * ddx + ddy * ddy) * (1.0 - 1e-9));
             }
         };
         // squared distance to candidate p, or -1 when the query does not see it
@@ -442,12 +448,12 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
 
         // ---- phase 1: the KMAX smallest values
         int rlast = 0;
-        for (int r = 0; r <= rlimit; ++r) {
+        for (int r = 0; r <= rlimit; r = r ? r + 1 : 2) {
             rlast = r;
-            ring(r, [&](int c, double md) {
+            ring(r, [&](int c0, int c1, double md) {
                 if (md > kv[KMAX - 1]) return;
-                const int p1 = cell_start[c + 1];
-                for (int p = cell_start[c]; p < p1; ++p) {
+                const int p1 = cell_start[c1];
+                for (int p = cell_start[c0]; p < p1; ++p) {
                     const double d = dist(p);
                     if (d >= 0.0 && d < kv[KMAX - 1]) {
                         double cv = d;
@@ -456,13 +462,15 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
                     }
                 }
             });
-            const double bx0 = qx - (g.x0 + (double)(qcx - r) * g.csx);
-            const double bx1 = (g.x0 + (double)(qcx + r + 1) * g.csx) - qx;
-            const double by0 = qy - (g.y0 + (double)(qcy - r) * g.csy);
-            const double by1 = (g.y0 + (double)(qcy + r + 1) * g.csy) - qy;
+            // every ring up to rb is walked
+            const int rb = r ? r : 1;
+            const double bx0 = qx - (g.x0 + (double)(qcx - rb) * g.csx);
+            const double bx1 = (g.x0 + (double)(qcx + rb + 1) * g.csx) - qx;
+            const double by0 = qy - (g.y0 + (double)(qcy - rb) * g.csy);
+            const double by1 = (g.y0 + (double)(qcy + rb + 1) * g.csy) - qy;
             const double b = fmin(fmin(bx0, bx1), fmin(by0, by1));
             if (b > 0.0 && b * b * (1.0 - 1e-9) > kv[KMAX - 1]) break;
-            if (qcx - r <= 0 && qcy - r <= 0 && qcx + r >= g.gx - 1 && qcy + r >= g.gy - 1) break;
+            if (qcx - rb <= 0 && qcy - rb <= 0 && qcx + rb >= g.gx - 1 && qcy + rb >= g.gy - 1) break;
         }
 
         // ---- phase 2: every candidate with d <= tau, at its rank
@@ -483,11 +491,11 @@ __global__ __launch_bounds__(256) void knn_select_kernel(const GridParams *__res
         };
         int nsel = 0;
         if (!tie) {
-            for (int r = 0; r <= rlast; ++r) {
-                ring(r, [&](int c, double md) {
+            for (int r = 0; r <= rlast; r = r ? r + 1 : 2) {
+                ring(r, [&](int c0, int c1, double md) {
                     if (md > tau) return;
-                    const int p1 = cell_start[c + 1];
-                    for (int p = cell_start[c]; p < p1; ++p) {
+                    const int p1 = cell_start[c1];
+                    for (int p = cell_start[c0]; p < p1; ++p) {
                         const double d = dist(p);
                         if (d >= 0.0 && d <= tau) {
                             if (dense) {
