@@ -32,7 +32,7 @@ struct rbffd_buf {
 
 // Workspace slots of one neighbour search and the per-set buffers of a batch.
 enum ArenaSlot {
-    A_PART, A_BOX, A_GRID, A_COUNTS, A_START, A_CELL_OF, A_FILL, A_CELL_XY, A_CELL_IDX, A_CELL_GRP,
+    A_PART, A_BOX, A_GRID, A_COUNTS, A_START, A_CELL_OF, A_SLOT, A_CELL_XY, A_CELL_IDX, A_CELL_GRP,
     A_SCAN_TMP, A_STENCILS, A_STATUS, A_REDO, A_REDO_TAU, A_REDO2, A_GEN_SCRATCH, A_NSLOT
 };
 
@@ -360,7 +360,7 @@ void knn_enqueue(rbffd_arena &A, bool may_alloc, const double *X, long N, const 
     int *counts = (int *)grow(A.b[A_COUNTS], sizeof(int) * (ncell + 1), may_alloc);
     int *start = (int *)grow(A.b[A_START], sizeof(int) * (ncell + 1), may_alloc);
     int *cell_of = (int *)grow(A.b[A_CELL_OF], sizeof(int) * N, may_alloc);
-    int *fill = (int *)grow(A.b[A_FILL], sizeof(int) * ncell, may_alloc);
+    int *slot = (int *)grow(A.b[A_SLOT], sizeof(int) * N, may_alloc);
     double *cxy = (double *)grow(A.b[A_CELL_XY], sizeof(double) * 2 * N, may_alloc);
     int *cidx = (int *)grow(A.b[A_CELL_IDX], sizeof(int) * N, may_alloc);
     int *cgrp = xgroup ? (int *)grow(A.b[A_CELL_GRP], sizeof(int) * N, may_alloc) : nullptr;
@@ -370,10 +370,9 @@ void knn_enqueue(rbffd_arena &A, bool may_alloc, const double *X, long N, const 
     rbffd::launch_bbox(X, N, box, part, nb, s);
     rbffd::launch_grid_params(box, N, k, grid, flag, s);
     rbffd::launch_fill_i32(counts, ncell + 1, 0, s);
-    rbffd::launch_fill_i32(fill, ncell, 0, s);
-    rbffd::launch_cell_count(X, N, grid, cell_of, counts, s);
+    rbffd::launch_cell_count(X, N, grid, cell_of, counts, slot, s);
     rbffd::launch_exclusive_scan(counts, start, ncell + 1, tmp, tb, s);
-    rbffd::launch_cell_scatter(X, xgroup, N, cell_of, start, fill, cxy, cidx, cgrp, s);
+    rbffd::launch_cell_scatter(X, xgroup, N, cell_of, start, slot, cxy, cidx, cgrp, s);
     // X queries run in cell order so a wavefront walks neighbouring cells
     const int *qorder = (Q == X && M == N) ? cidx : nullptr;
     int *redo = (int *)grow(A.b[A_REDO], sizeof(int) * (M + 1), may_alloc);

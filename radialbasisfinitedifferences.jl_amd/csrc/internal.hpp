@@ -36,8 +36,10 @@ struct GridParams {
 // Bounding box of X (device): box[5] = (xmin, ymin, xmax, ymax, non-finite count);
 // partials holds 5 values per block.
 void launch_bbox(const double *X, long N, double *box, double *partials, int nblocks, hipStream_t s);
-// Cell index of every point and the per-cell counts (counts must be zeroed).
-void launch_cell_count(const double *X, long N, const GridParams *g, int *cell_of, int *counts, hipStream_t s);
+// Cell index of every point, the per-cell counts (counts must be zeroed) and
+// each point's slot inside its cell (slot: N ints).
+void launch_cell_count(const double *X, long N, const GridParams *g, int *cell_of, int *counts, int *slot,
+                       hipStream_t s);
 // Upper bound on the grid's cell count for N points.
 __host__ __device__ inline long grid_max_cells(long N) { return 4 * N + 64; }
 // Grid parameters computed on the device from box (see launch_bbox); bit 0 of
@@ -49,9 +51,9 @@ void launch_fill_i64(int64_t *p, long n, int64_t v, hipStream_t s);
 // Exclusive scan of counts (ncell) into start (ncell + 1); tmp as returned by scan_tmp_bytes.
 size_t scan_tmp_bytes(long n);
 void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tmp_bytes, hipStream_t s);
-// Scatter points into cell order (fill must be zeroed).
+// Scatter points into cell order: point i goes to start[cell_of[i]] + slot[i].
 void launch_cell_scatter(const double *X, const int *xgroup, long N, const int *cell_of, const int *start,
-                         int *fill, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s);
+                         const int *slot, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s);
 // Exact kNN of every query, ascending (d^2, index), over the N points of the
 // cell arrays; *unfilled (may be NULL) gains the result slots left at -1 (a
 // query that sees fewer than k nodes, or non-finite points).  redo (M ints, may be

@@ -84,8 +84,10 @@ __device__ __forceinline__ int cell_coord(double v, double v0, double cs, int g)
     return c >= g ? g - 1 : c;
 }
 
+// The count's atomic also hands each point its slot inside the cell, so the
+// scatter needs neither a second atomic nor a zeroed fill array.
 __global__ void cell_count_kernel(const double *X, long N, const GridParams *__restrict__ gp, int *cell_of,
-                                  int *counts)
+                                  int *counts, int *slot)
 {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -94,13 +96,14 @@ __global__ void cell_count_kernel(const double *X, long N, const GridParams *__r
     int cy = cell_coord(X[2 * i + 1], g.y0, g.csy, g.gy);
     int c = cy * g.gx + cx;
     cell_of[i] = c;
-    atomicAdd(&counts[c], 1);
+    slot[i] = atomicAdd(&counts[c], 1);
 }
 
-void launch_cell_count(const double *X, long N, const GridParams *gp, int *cell_of, int *counts, hipStream_t s)
+void launch_cell_count(const double *X, long N, const GridParams *gp, int *cell_of, int *counts, int *slot,
+                       hipStream_t s)
 {
     if (N <= 0) return;
-    cell_count_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, N, gp, cell_of, counts);
+    cell_count_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, N, gp, cell_of, counts, slot);
 }
 
 // Grid of about ppc points per cell over the bounding box, at most
@@ -175,13 +178,12 @@ void launch_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t tm
 }
 
 __global__ void cell_scatter_kernel(const double *X, const int *xgroup, long N, const int *cell_of,
-                                    const int *start, int *fill, double *cell_xy, int *cell_idx,
+                                    const int *start, const int *slot, double *cell_xy, int *cell_idx,
                                     int *cell_grp)
 {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= N) return;
-    int c = cell_of[i];
-    int pos = start[c] + atomicAdd(&fill[c], 1);
+    const int pos = start[cell_of[i]] + slot[i];
     cell_xy[2 * (long)pos] = X[2 * i];
     cell_xy[2 * (long)pos + 1] = X[2 * i + 1];
     cell_idx[pos] = (int)i;
@@ -189,10 +191,10 @@ __global__ void cell_scatter_kernel(const double *X, const int *xgroup, long N, 
 }
 
 void launch_cell_scatter(const double *X, const int *xgroup, long N, const int *cell_of, const int *start,
-                         int *fill, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s)
+                         const int *slot, double *cell_xy, int *cell_idx, int *cell_grp, hipStream_t s)
 {
     if (N <= 0) return;
-    cell_scatter_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, xgroup, N, cell_of, start, fill,
+    cell_scatter_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(X, xgroup, N, cell_of, start, slot,
                                                                      cell_xy, cell_idx, cell_grp);
 }
 
