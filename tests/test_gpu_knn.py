@@ -29,6 +29,9 @@ def _sets():
         "clustered": np.concatenate([rng.normal(0.3, 0.01, size=(800, 2)), rng.normal(0.6, 0.2, size=(900, 2))]),
         "duplicates": np.repeat(rng.uniform(0, 1, size=(600, 2)), 2, axis=0),
         "line": np.stack([np.linspace(0, 1, 500), np.full(500, 0.25)], axis=1),
+        # one cell per grid row, and a few: the row runs of the cell walk clamp at both ends
+        "vertical_line": np.stack([np.full(400, 0.5), np.linspace(0, 2, 400)], axis=1),
+        "strip": np.stack([rng.uniform(0, 0.02, size=1500), rng.uniform(0, 1, size=1500)], axis=1),
     }
 
 
