@@ -22,7 +22,10 @@ where Julia raises ArgumentError (e.g. n larger than the node count).
 from __future__ import annotations
 
 import ctypes
+import os
+import re
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import scipy.sparse as sp
@@ -50,6 +53,49 @@ def _pts(a):
 
 def _vp(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def host_threads() -> int:
+    """Threads for large host copies: RBFFD_HOST_THREADS, else OMP_NUM_THREADS
+    (a positive leading integer), else the CPUs this process may run on; at
+    most 32.  The same rule as the C library's device-to-host copy pool."""
+    for name in ("RBFFD_HOST_THREADS", "OMP_NUM_THREADS"):
+        m = re.match(r"\s*([+-]?\d+)", os.environ.get(name, ""))
+        if m and int(m.group(1)) > 0:
+            return min(32, int(m.group(1)))
+    try:
+        return max(1, min(32, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(32, os.cpu_count() or 1))
+
+
+_copy_pool = (None, 0)
+
+
+def _host_copy(a: np.ndarray, dtype=None) -> np.ndarray:
+    """A copy of the contiguous 1-D array a, converted to dtype when given.
+    Large copies are split over host_threads() threads (numpy releases the GIL
+    while it copies): touching the new pages for the first time costs as much
+    as the copy itself."""
+    global _copy_pool
+    out = np.empty(a.shape, a.dtype if dtype is None else dtype)
+    T = host_threads()
+    if T == 1 or a.nbytes < (16 << 20):
+        np.copyto(out, a)
+        return out
+    pool, size = _copy_pool
+    if pool is None or size != T:
+        if pool is not None:
+            pool.shutdown(wait=True)
+        pool = ThreadPoolExecutor(max_workers=T)
+        _copy_pool = (pool, T)
+    edges = [len(a) * i // T for i in range(T + 1)]
+
+    def part(i):
+        np.copyto(out[edges[i]:edges[i + 1]], a[edges[i]:edges[i + 1]])
+
+    list(pool.map(part, range(T)))
+    return out
 
 
 class _Boundary:
@@ -126,9 +172,14 @@ def operators(X, Y, p, n, polydeg, ops, k=0, boundary=None, precision="fp64", ct
     nc = int(ncols.value)
     # every matrix owns its index arrays, as the reference's separate sparse()
     # calls return independent matrices (src/generate_operator.jl:176-181): a
-    # structural edit of one leaves the others intact
-    cp = colptr[: nc + 1]
-    mats = tuple(sp.csc_matrix((nz[o], rowval if o == 0 else rowval.copy(), cp if o == 0 else cp.copy()),
+    # structural edit of one leaves the others intact.  scipy keeps 32-bit
+    # indices whenever they fit, and converting int64 ones itself costs a scan
+    # and a single-threaded copy per matrix, so the indices are converted here,
+    # once, and copied per matrix with the host threads.
+    idx_t = np.int32 if max(nnz, M, nc) <= np.iinfo(np.int32).max else np.int64
+    rv = _host_copy(rowval, idx_t)
+    cp = colptr[: nc + 1].astype(idx_t)
+    mats = tuple(sp.csc_matrix((nz[o], rv if o == 0 else _host_copy(rv), cp if o == 0 else cp.copy()),
                                shape=(M, nc), copy=False) for o in range(nops))
     if return_stats:
         st = stats.as_dict()
