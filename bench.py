@@ -362,6 +362,7 @@ def spmv_only():
     N, n = len(X), prm["n"]
     Xd = ctx.to_device(X)
     col, vals, _ = eng.generate(Xd, Xd, prm["p"], n, prm["polydeg"], ["Lap"])
+    morton = eng.locality_order(Xd)
     out = {}
     for F in (1, 8):
         x = ctx.to_device(np.random.default_rng(F).standard_normal((N, F)))
@@ -371,6 +372,14 @@ def spmv_only():
         for _ in range(10):
             eng.spmv(col, vals[0], x, y, uniform_n=n)
         out[f"F{F}_ms"] = ctx.timer_stop() / 10
+        # the same products through an application plan
+        plan = eng.spmv_plan(col, vals[0], N, N, uniform_n=n, order=morton)
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(10):
+            plan.apply(x, y)
+        out[f"F{F}_plan_ms"] = ctx.timer_stop() / 10
+        plan.close()
     print(json.dumps(out))
 
 
