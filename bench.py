@@ -96,8 +96,8 @@ def main():
                     help="comma-separated subset of the extra workloads: hyperviscosity_4m, uneven_batch_64, "
                          "advdiff_ssprk3_4m, poisson_solve, y_distinct_4m (default all)")
     ap.add_argument("--spmv-only", action="store_true",
-                    help="profiling workload: generate the Laplacian of the 1M-node set, then 10 products "
-                         "with 1 field and 10 with 8 fields; prints their mean times")
+                    help="profiling workload: generate the Laplacian of the 1M-node set, then for 1 and 8 "
+                         "fields 10 plain products and 10 through an application plan; prints their mean times")
     args = ap.parse_args()
     if args.spmv_only:
         spmv_only()
